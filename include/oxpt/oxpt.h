@@ -1,0 +1,330 @@
+/*
+ * oxpt.h — C ABI of liboxpt, the MI355X (gfx950) path tracer.
+ *
+ * Two layers, both `extern "C"`, plain pointers and sizes:
+ *
+ *  1. The device boundary (oxpt_ctx / oxpt_scene / oxpt_accum): the HIP
+ *     kernels behind opaque handles. This is what the reference's GPU
+ *     backend does through wgpu (oxideav-render-vulkan src/pathtrace.rs);
+ *     a Rust `-sys` crate would bind exactly these declarations.
+ *  2. The host tracer (oxpt_tracer): the progressive `GpuPathTracer` API
+ *     with the reference's reset rules (src/pathtrace.rs:309-725), built
+ *     in C++ on top of layer 1, plus the scene builders it consumes.
+ *
+ * Conventions (src/pathtrace.rs, src/gpu.rs):
+ *  - every call returns an int status: OXPT_OK (0) or a negative OXPT_E_*;
+ *    the message is available from oxpt_last_error / oxpt_tracer_last_error
+ *    (the reference's Error::Backend(String) / Error::InvalidOptions text);
+ *  - the caller owns host buffers, the library owns device memory;
+ *  - one handle per thread, or external locking (the reference's `&mut self`);
+ *    device opening is serialised process-wide (src/gpu.rs:978-992);
+ *  - refine is asynchronous (stream-ordered); resolve / image / hdr / finish
+ *    block (src/pathtrace.rs:578-583, src/gpu.rs:1095-1107).
+ */
+#ifndef OXPT_OXPT_H
+#define OXPT_OXPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "oxpt/layout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define OXPT_OK 0
+#define OXPT_E_BACKEND -1  /* Error::Backend: device, limits, allocation, launch */
+#define OXPT_E_INVALID -2  /* Error::InvalidOptions: e.g. oversize dimensions */
+#define OXPT_E_ARG -3      /* bad argument (null handle, short buffer) */
+
+/* =====================================================================
+ * Layer 1: device boundary.
+ * ===================================================================== */
+
+typedef struct oxpt_ctx oxpt_ctx;
+typedef struct oxpt_scene oxpt_scene;
+typedef struct oxpt_accum oxpt_accum;
+
+/* Packed scene words (include/oxpt/layout.h), copied at upload. Replaces the
+ * buffer creation in GpuPathTracer::upload / ensure_binding
+ * (src/pathtrace.rs:970-1239, 847-966). */
+typedef struct oxpt_scene_desc {
+    const float *geom;       /* float4 words */
+    size_t geom_vec4;
+    const float *attrs;      /* float4 words */
+    size_t attrs_vec4;
+    const uint32_t *tables;
+    size_t table_words;
+    const uint16_t *atlas;   /* binary16 RGBA texels, may be NULL */
+    uint32_t atlas_side;
+    uint32_t atlas_layers;
+} oxpt_scene_desc;
+
+/* Opens device `device_ordinal` (open_device_blocking, src/gpu.rs:987-1056).
+ * OXPT_E_BACKEND when no such HIP device exists. */
+int oxpt_ctx_create(int device_ordinal, oxpt_ctx **out);
+void oxpt_ctx_destroy(oxpt_ctx *ctx);
+/* Text of the last failure on this context ("" if none). */
+const char *oxpt_last_error(const oxpt_ctx *ctx);
+/* "<device name> (gfx arch, HIP)" — adapter_summary (src/pathtrace.rs:464-467). */
+const char *oxpt_ctx_summary(const oxpt_ctx *ctx);
+/* Device memory budget in bytes that uploads and accumulators may use
+ * (default: the device's total memory); allocation past it fails with
+ * OXPT_E_BACKEND, as check_size does (src/pathtrace.rs:567-576). */
+int oxpt_ctx_set_budget(oxpt_ctx *ctx, uint64_t bytes);
+uint64_t oxpt_ctx_bytes_in_use(const oxpt_ctx *ctx);
+
+int oxpt_scene_upload(oxpt_ctx *ctx, const oxpt_scene_desc *desc, oxpt_scene **out);
+/* Replace the table words only (environment change, ensure_binding,
+ * src/pathtrace.rs:847-869); geometry and atlas stay resident. */
+int oxpt_scene_set_tables(oxpt_ctx *ctx, oxpt_scene *scene, const uint32_t *tables, size_t words);
+void oxpt_scene_destroy(oxpt_scene *scene);
+/* Device bytes held by a scene. */
+uint64_t oxpt_scene_bytes(const oxpt_scene *scene);
+
+/* Accumulator of w*h float4 (src/pathtrace.rs:551-561), zeroed. */
+int oxpt_accum_create(oxpt_ctx *ctx, uint32_t w, uint32_t h, oxpt_accum **out);
+void oxpt_accum_destroy(oxpt_accum *acc);
+/* reset(): clear to zero (src/pathtrace.rs:495-502). */
+int oxpt_accum_clear(oxpt_ctx *ctx, oxpt_accum *acc);
+/* Copy the raw accumulator (w*h*16 bytes) to host, blocking. */
+int oxpt_accum_read(oxpt_ctx *ctx, const oxpt_accum *acc, float *host_out, size_t host_bytes);
+/* Overwrite the raw accumulator from host (w*h*16 bytes). */
+int oxpt_accum_write(oxpt_ctx *ctx, oxpt_accum *acc, const float *host_in, size_t host_bytes);
+
+/* Trace samples [first, first + count) of every pixel and add them to the
+ * accumulator, in sample order per pixel; asynchronous. Replaces the
+ * refine dispatch loop (src/pathtrace.rs:584-671) and trace_main
+ * (src/shaders/pathtrace.wgsl:2047-2073). `P` is the 176-byte Params. */
+int oxpt_refine(oxpt_ctx *ctx, const oxpt_scene *scene, oxpt_accum *acc, const oxpt_params *P,
+                uint32_t first, uint32_t count);
+/* Block until every queued launch finished (finish, src/pathtrace.rs:673-679). */
+int oxpt_finish(oxpt_ctx *ctx);
+/* Resolve into host memory, blocking: RGBA8 when rp->cfg[3] == 0 (w*h*4
+ * bytes), RGBA32F when rp->cfg[3] == 1 (w*h*16 bytes). Replaces
+ * encode_resolve + readback_texture (src/pathtrace.rs:769-844,
+ * src/shaders/resolve.wgsl:151-188, src/gpu.rs:1060-1117). */
+int oxpt_resolve(oxpt_ctx *ctx, const oxpt_accum *acc, const oxpt_resolve_params *rp,
+                 void *host_out, size_t host_bytes);
+
+/* Kernel timing: HIP events around every trace launch on the context's
+ * stream. Returns launches and total milliseconds since the last reset. */
+int oxpt_ctx_timing(oxpt_ctx *ctx, int reset, uint64_t *launches, double *total_ms);
+
+/* Launch shape and traversal knobs (autotuner). 0 keeps the default. */
+typedef struct oxpt_tuning {
+    uint32_t block;          /* threads per workgroup: 64, 128, 256 */
+    uint32_t waves_per_cu;   /* persistent residency: workgroups = CUs * this / waves per WG */
+    uint32_t max_paths;      /* paths per launch cap (samples are split, order kept) */
+    uint32_t kernel;         /* 0 = persistent megakernel, 1 = plain one-pixel-per-thread */
+} oxpt_tuning;
+int oxpt_ctx_set_tuning(oxpt_ctx *ctx, const oxpt_tuning *t);
+int oxpt_ctx_get_tuning(const oxpt_ctx *ctx, oxpt_tuning *t);
+
+/* Debug / parity probes (no counterpart in the reference: they expose the
+ * kernel's integer sampler and light-CDF picks for bit-exact checks).
+ * Pattern words: for n tuples (x, y, index, p) the 4 scrambled Sobol' words
+ * of pattern p (pathtrace.wgsl:202-224) before to_unit. */
+int oxpt_debug_pattern_words(oxpt_ctx *ctx, const oxpt_scene *scene, uint32_t seed, size_t n,
+                             const uint32_t *x, const uint32_t *y, const uint32_t *index,
+                             const uint32_t *p, uint32_t *out4);
+/* Light picks for n uniforms: {emissive index (pathtrace.wgsl:1466-1484),
+ * env row, env column (pathtrace.wgsl:1633-1635)}. */
+int oxpt_debug_light_picks(oxpt_ctx *ctx, const oxpt_scene *scene, const oxpt_params *P, size_t n,
+                           const float *u, uint32_t *out3);
+/* Closest hit of n rays (o, d float3 each): {slot, t bits}. */
+int oxpt_debug_closest_hit(oxpt_ctx *ctx, const oxpt_scene *scene, const oxpt_params *P, size_t n,
+                           const float *o3, const float *d3, uint32_t *out2);
+
+/* =====================================================================
+ * Layer 2: host tracer and scenes.
+ * ===================================================================== */
+
+typedef struct oxpt_host_scene oxpt_host_scene;
+typedef struct oxpt_env_map oxpt_env_map;
+typedef struct oxpt_tracer oxpt_tracer;
+
+/* Texture binding of a material slot (KHR_texture_transform as a 2x3). */
+typedef struct oxpt_tex_binding {
+    int32_t texture;  /* index into the scene's textures, -1 = none */
+    uint32_t uv_set;  /* 0 or 1 */
+    float xform[6];   /* rows (a, b, c), (d, e, f): u' = a u + b v + c */
+} oxpt_tex_binding;
+
+/* glTF metallic-roughness material with the KHR extensions the estimator
+ * reads (src/pathtrace.rs:1338-1423). */
+typedef struct oxpt_material {
+    float base_color[4];
+    float emissive[3];
+    uint32_t alpha_mode;   /* 0 opaque, 1 mask, 2 blend */
+    float alpha_cutoff;
+    uint32_t double_sided;
+    uint32_t unlit;
+    float metallic, roughness, normal_scale, ior;
+    uint32_t has_specular; float specular, specular_color[3];
+    uint32_t has_transmission; float transmission;
+    uint32_t has_volume; float thickness, attenuation_color[3], attenuation_distance; /* <= 0: infinite */
+    uint32_t has_clearcoat; float clearcoat, clearcoat_roughness, clearcoat_normal_scale;
+    uint32_t has_sheen; float sheen_color[3], sheen_roughness;
+    oxpt_tex_binding tex[OXPT_N_TEX_SLOTS]; /* slot order as OXPT_S_* */
+} oxpt_material;
+
+/* glTF sampler: wrap 0 clamp, 1 mirrored repeat, 2 repeat; mag 0 nearest,
+ * 1 linear; min 0..5 = Nearest, Linear, NMN, LMN, NML, LML. */
+typedef struct oxpt_sampler {
+    uint32_t wrap_s, wrap_t, mag, min;
+} oxpt_sampler;
+
+/* Punctual light (KHR_lights_punctual). kind 0 directional, 1 point, 2 spot. */
+typedef struct oxpt_light {
+    uint32_t kind;
+    float position[3], direction[3];
+    float color[3], intensity;
+    float range;  /* <= 0: none */
+    float inner_cone, outer_cone;
+} oxpt_light;
+
+/* Camera: pinhole (yfov radians) or orthographic (ymag half-height). */
+typedef struct oxpt_camera {
+    float eye[3], target[3], up[3];
+    float yfov;
+    uint32_t orthographic;
+    float ymag;
+} oxpt_camera;
+
+/* Render options: the fields of RenderOptions + PathTraceOptions the path
+ * tracer reads (src/pathtrace.rs:885-926, 782-791). */
+typedef struct oxpt_render_options {
+    uint32_t width, height;
+    uint32_t samples_per_pixel;
+    uint32_t max_bounces;
+    uint32_t rr_start;
+    uint32_t strategy;   /* 0 MIS, 1 light only, 2 BSDF only */
+    uint32_t seed;
+    float clamp;         /* 0 = off */
+    float ambient;
+    uint8_t background[4];
+    uint32_t tone_map;   /* 0 clamp, 1 Reinhard, 2 ACES fitted */
+    float exposure;
+} oxpt_render_options;
+
+/* Defaults: 256x256, 64 spp, 8 bounces, roulette from 3, MIS, seed 0, no
+ * clamp, ambient 0, background (16,16,20,255), clamp tone map, exposure 1. */
+void oxpt_render_options_default(oxpt_render_options *o);
+
+oxpt_host_scene *oxpt_host_scene_new(void);
+void oxpt_host_scene_free(oxpt_host_scene *s);
+int oxpt_host_scene_add_material(oxpt_host_scene *s, const oxpt_material *m);
+/* Material with glTF defaults (white, metallic 1, roughness 1, ior 1.5). */
+void oxpt_material_default(oxpt_material *m);
+/* RGBA8 texture (level 0); mips are built at pack time. srgb_hint unused by
+ * the estimator (each slot picks its colour space). Returns the index. */
+int oxpt_host_scene_add_texture(oxpt_host_scene *s, uint32_t w, uint32_t h, const uint8_t *rgba8,
+                                const oxpt_sampler *sampler);
+/* Append n triangles: positions 9 floats each; optional per-corner arrays
+ * (normals 9, tangents 12, uv0 6, uv1 6, colors 12 floats per triangle;
+ * NULL = absent); one material index per triangle. */
+int oxpt_host_scene_add_triangles(oxpt_host_scene *s, size_t n, const float *pos, const float *normals,
+                                  const float *tangents, const float *uv0, const float *uv1,
+                                  const float *colors, const uint32_t *material);
+int oxpt_host_scene_add_light(oxpt_host_scene *s, const oxpt_light *l);
+int oxpt_host_scene_set_camera(oxpt_host_scene *s, const oxpt_camera *c);
+size_t oxpt_host_scene_triangle_count(const oxpt_host_scene *s);
+
+/* Equirectangular environment (EnvironmentMap): w*h RGB f32 texels. */
+oxpt_env_map *oxpt_env_new(uint32_t w, uint32_t h, const float *rgb, float intensity);
+void oxpt_env_free(oxpt_env_map *e);
+
+/* Seeded scene generators. */
+/* Closed room of 5 quads + 2 rotated boxes + emissive ceiling quad (Le 15),
+ * Lambert colours U(0.2, 0.8), 40 degree pinhole camera. */
+oxpt_host_scene *oxpt_testscene_room(uint32_t seed);
+/* Double-sided emissive Lambert cube [-1,1]^3 seen from inside
+ * (tests/pathtrace.rs:155-170): L = Le / (1 - rho). */
+oxpt_host_scene *oxpt_testscene_furnace(float rho, float le);
+/* fBm heightfield (gx x gz quads, 2 tris each) + n_spheres icospheres
+ * (subdivision `subdiv`), glTF MR materials (metallic ~ Bernoulli(0.3),
+ * roughness ~ U(0.05, 1)), n_emissive emissive triangles. */
+oxpt_host_scene *oxpt_testscene_terrain(uint32_t seed, uint32_t gx, uint32_t gz, uint32_t n_spheres,
+                                        uint32_t subdiv, uint32_t n_emissive);
+/* Procedural sky: gradient + sun, w x h. */
+oxpt_env_map *oxpt_env_sky(uint32_t w, uint32_t h, uint32_t seed);
+/* Layered-material scene: instanced meshes baked to ~n_tris triangles,
+ * transmission+volume / clearcoat / sheen / BLEND mixes, n_textures
+ * value-noise RGBA textures of tex_size^2 with mips, n_lights punctual
+ * lights. */
+oxpt_host_scene *oxpt_testscene_layered(uint32_t seed, uint32_t n_tris, uint32_t n_textures,
+                                        uint32_t tex_size, uint32_t n_lights);
+
+/* Progressive tracer (GpuPathTracer). */
+/* GpuPathTracer::new — opens device 0. */
+int oxpt_tracer_new(oxpt_tracer **out);
+/* Tracer on device `ordinal` (with_backend / from_device). */
+int oxpt_tracer_new_on(int ordinal, oxpt_tracer **out);
+void oxpt_tracer_free(oxpt_tracer *t);
+const char *oxpt_tracer_last_error(const oxpt_tracer *t);
+const char *oxpt_tracer_adapter_summary(const oxpt_tracer *t);
+/* sync: *reset = 1 iff the accumulation was reset (src/pathtrace.rs:526-565). */
+int oxpt_tracer_sync(oxpt_tracer *t, const oxpt_host_scene *scene, const oxpt_render_options *o,
+                     int *reset);
+int oxpt_tracer_refine(oxpt_tracer *t, uint32_t samples);
+int oxpt_tracer_finish(oxpt_tracer *t);
+int oxpt_tracer_image(oxpt_tracer *t, uint8_t *out_rgba8, size_t bytes);
+int oxpt_tracer_hdr(oxpt_tracer *t, float *out_rgba32f, size_t bytes);
+void oxpt_tracer_reset(oxpt_tracer *t);
+uint32_t oxpt_tracer_samples(const oxpt_tracer *t);
+int oxpt_tracer_is_converged(const oxpt_tracer *t);
+void oxpt_tracer_size(const oxpt_tracer *t, uint32_t *w, uint32_t *h);
+void oxpt_tracer_invalidate_scene(oxpt_tracer *t);
+/* set_environment: NULL removes it; resets the accumulation. The map is copied. */
+int oxpt_tracer_set_environment(oxpt_tracer *t, const oxpt_env_map *env);
+/* Renderer::render / render_hdr (src/pathtrace.rs:1241-1259). */
+int oxpt_tracer_render(oxpt_tracer *t, const oxpt_host_scene *scene, const oxpt_render_options *o,
+                       uint8_t *out_rgba8, size_t bytes);
+int oxpt_tracer_render_hdr(oxpt_tracer *t, const oxpt_host_scene *scene,
+                           const oxpt_render_options *o, float *out_rgba32f, size_t bytes);
+/* The device context (for timing, tuning and the debug probes) and the
+ * current device scene / params (valid after sync + one refine). */
+oxpt_ctx *oxpt_tracer_ctx(oxpt_tracer *t);
+const oxpt_scene *oxpt_tracer_device_scene(oxpt_tracer *t);
+int oxpt_tracer_params(oxpt_tracer *t, oxpt_params *out);
+/* Host copies of the packed words the kernels read (for the CPU oracle):
+ * which = 0 geom (float4 count), 1 attrs (float4 count), 2 tables (words,
+ * environment included), 3 atlas (u16 count). */
+int oxpt_tracer_words(oxpt_tracer *t, int which, const void **ptr, size_t *count);
+int oxpt_tracer_atlas_shape(oxpt_tracer *t, uint32_t *side, uint32_t *layers);
+
+/* Host-only packing (no device): the words and Params a sync + refine would
+ * hand the kernels (upload + ensure_binding, src/pathtrace.rs:847-1239).
+ * which: as oxpt_tracer_words. */
+typedef struct oxpt_packed oxpt_packed;
+int oxpt_pack(const oxpt_host_scene *scene, const oxpt_env_map *env, const oxpt_render_options *o,
+              oxpt_packed **out);
+void oxpt_packed_free(oxpt_packed *p);
+int oxpt_packed_words(const oxpt_packed *p, int which, const void **ptr, size_t *count);
+int oxpt_packed_params(const oxpt_packed *p, oxpt_params *out);
+int oxpt_packed_info(const oxpt_packed *p, uint32_t *atlas_side, uint32_t *atlas_layers, uint32_t *bvh_depth);
+
+/* Registry (register_into, src/lib.rs:117,329-338). */
+#define OXPT_PATHTRACE_BACKEND_NAME "gpu-pathtrace"
+/* Factory: a tracer for a registered name, OXPT_E_INVALID for unknown names. */
+int oxpt_registry_make(const char *name, oxpt_tracer **out);
+
+/* sizeof of the ABI structs, for binding checks: params, frame,
+ * resolve_params, scene_desc, tex_binding, material, sampler, light, camera,
+ * render_options, tuning (11 entries). */
+void oxpt_abi_sizes(uint32_t out[11]);
+
+/* Pack helper exposed for the f16 KAT (src/scene.rs:244-272). */
+uint16_t oxpt_f32_to_f16(float v);
+/* The packed Sobol' table and sheen LUT (product copies). */
+void oxpt_sobol_directions(uint32_t out[OXPT_SOBOL_WORDS]);
+void oxpt_sheen_table(float out[OXPT_SHEEN_WORDS]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OXPT_OXPT_H */

@@ -1,0 +1,623 @@
+// Path-trace kernels for gfx950 (CDNA4, wave64).
+//
+// trace_persistent — the refine step (oxideav-render-vulkan
+//   src/shaders/pathtrace.wgsl:1714-2073 trace_sample + trace_main). Each
+//   lane owns one pixel at a time and traces that pixel's samples
+//   [first, first + count) in order, adding them into the pixel's float4
+//   accumulator exactly as the reference does, so results do not depend on
+//   scheduling. The path is a state machine that traces exactly one ray per
+//   loop iteration (the continuation ray or one NEE shadow ray), and a lane
+//   whose pixel is done takes the next one at once: idle lanes of a wave are
+//   counted with a ballot, one lane claims that many pixels with a single
+//   atomic on the work counter, and each idle lane takes its rank among
+//   them. Work items are ordered in 8x8 pixel tiles so a fresh wave starts
+//   on one coherent tile.
+// trace_tiles — the plain form: one thread per pixel, 8x8 tiles per wave,
+//   no regeneration (the reference's dispatch shape).
+// resolve_pt — accumulator -> RGBA8 / RGBA32F (src/shaders/resolve.wgsl:151-188).
+// debug_* — integer sampler words, light-CDF picks and closest hits for the
+//   bit-exact parity probes.
+#include "pt_kernels.hpp"
+#include "pt_shade.hpp"
+
+namespace oxd {
+
+__device__ __forceinline__ Scene make_scene(const TraceArgs &A, const float *lds_sheen) {
+    Scene S;
+    S.geom = A.geom;
+    S.attrs = A.attrs;
+    S.tables = A.tables;
+    S.atlas = A.atlas;
+    S.atlas_side = A.atlas_side;
+    S.tri_base = A.P.offs2[2];
+    S.node_count = A.P.env[3];
+    S.off_mat = A.P.offs0[2];
+    S.off_plights = A.P.offs0[3];
+    S.off_elights = A.P.offs1[0];
+    S.n_emissive = A.P.cfg[3];
+    S.env_w = A.P.env[0];
+    S.env_h = A.P.env[1];
+    S.env_marg = A.P.offs1[2];
+    S.env_cond = A.P.offs1[3];
+    S.env_mass = A.P.offs2[0];
+    S.env_rad = A.P.offs2[1];
+    S.p_env = A.P.fparams[2];
+    S.lds_sheen = lds_sheen;
+    return S;
+}
+
+__device__ __forceinline__ F3 clamp_c(float cl, F3 c) {
+    float m = max3(c);
+    if (cl > 0.0f && m > cl) return c * (cl / m);
+    return c;
+}
+
+// Per-lane path state of the one-ray-per-iteration state machine
+// (wgsl:1733-1764).
+struct PathState {
+    uint32_t pixel_seed, path_seed, index;
+    F3 d, beta, radiance, prev_pos, sigma;
+    float prev_pdf;
+    bool in_medium;
+    uint32_t k;
+    bool shadow;
+    F3 r_o, r_d;
+    float r_tmax;
+    uint32_t r_id, r_skip;
+    F3 pending;
+    Bsdf bsdf;
+    F3 p, ng;
+    float thickness, transmission;
+    F3 atten_color;
+    float atten_dist;
+    bool front;
+    float4 pl_u;
+    uint32_t job;
+};
+
+// Camera ray of sample `index` of pixel (x, y) (wgsl:1715-1764).
+__device__ __forceinline__ void init_path(const TraceArgs &A, const uint32_t *lds_sobol, uint32_t x, uint32_t y,
+                                          uint32_t index, PathState &st) {
+    const oxpt_params &P = A.P;
+    st.pixel_seed = hash_mix(hash_mix(P.dims[2], y), x);
+    st.index = index;
+    st.path_seed = hash_mix(st.pixel_seed, index);
+    float4 p0 = pattern(lds_sobol, st.pixel_seed, index, 0);
+    float width = (float)P.dims[0], height = (float)P.dims[1];
+    float ndc_x = (((float)x + p0.x) / fmaxf(width, 1.0f)) * 2.0f - 1.0f;
+    float ndc_y = 1.0f - (((float)y + p0.y) / fmaxf(height, 1.0f)) * 2.0f;
+    F3 eye = {P.eye[0], P.eye[1], P.eye[2]};
+    F3 fwd = {P.forward[0], P.forward[1], P.forward[2]};
+    F3 side = {P.side[0], P.side[1], P.side[2]};
+    F3 up = {P.up[0], P.up[1], P.up[2]};
+    F3 o, d;
+    if (P.eye[3] > 0.5f) {
+        o = eye + (side * (ndc_x * P.forward[3]) + up * (ndc_y * P.side[3]));
+        d = fwd;
+    } else {
+        o = eye;
+        d = nrm(side * (ndc_x * P.forward[3]) + up * (ndc_y * P.side[3]) + fwd);
+    }
+    st.d = d;
+    st.beta = f3(1.0f);
+    st.radiance = f3(0.0f);
+    st.prev_pdf = 0.0f;
+    st.prev_pos = f3(0.0f);
+    st.in_medium = false;
+    st.sigma = f3(0.0f);
+    st.k = 0;
+    st.shadow = false;
+    st.r_o = o;
+    st.r_d = d;
+    st.r_tmax = INF;
+    st.r_id = 0;
+    st.r_skip = NONE;
+    st.pending = f3(0.0f);
+    st.p = f3(0.0f);
+    st.ng = f3(0.0f);
+    st.thickness = 0.0f;
+    st.transmission = 0.0f;
+    st.atten_color = f3(1.0f);
+    st.atten_dist = -1.0f;
+    st.front = true;
+    st.pl_u = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    st.job = 0;
+}
+
+// One iteration: trace the current ray and advance the state machine
+// (wgsl:1766-2040). Returns true when the path ended; *covered says
+// whether the camera ray hit anything.
+__device__ bool step_path(const TraceArgs &A, const Scene &S, const uint32_t *lds_sobol, PathState &st, bool *covered) {
+    const oxpt_params &P = A.P;
+    const uint32_t strategy = P.cfg[1];
+    const bool has_env = P.env[2] != 0;
+    const uint32_t n_elights = P.cfg[3], n_punct = P.cfg[2];
+    const float clampv = P.fparams[1];
+    *covered = true;
+    Hit hit = traverse(S, make_ray(st.r_o, st.r_d), st.r_tmax, st.path_seed, st.r_id, !st.shadow && st.k == 0,
+                       st.shadow, st.r_skip);
+    if (st.shadow) {
+        if (hit.slot == NONE) st.radiance = st.radiance + st.pending;
+        st.job += 1;
+    } else {
+        if (st.in_medium) {
+            if (hit.slot == NONE) return true;
+            F3 e = -st.sigma * hit.t;
+            st.beta = st.beta * F3{expf(e.x), expf(e.y), expf(e.z)};
+        }
+        if (hit.slot == NONE) {
+            if (st.k == 0) {
+                *covered = false;
+                return true;
+            }
+            F3 le = f3(P.fparams[0]);
+            float w = 1.0f;
+            if (has_env) {
+                Texel t = env_texel(S, st.d);
+                le = env_radiance_t(S, t);
+                if (strategy == 0) w = power_heuristic(st.prev_pdf, S.p_env * env_pdf_t(S, t));
+                else if (strategy == 1) w = 0.0f;
+            }
+            if (w > 0.0f) st.radiance = st.radiance + clamp_c(clampv, st.beta * le * w);
+            return true;
+        }
+        Surface surf = surface(S, hit.slot, hit.b);
+        uint32_t mflags = mat_flags(S, tri_material(S, hit.slot));
+        bool double_sided = (mflags & 4u) != 0;
+        Lod lod = base_lod();
+        if (st.k == 0) {
+            float cw = P.up[3];
+            if (!(P.eye[3] > 0.5f)) cw = P.up[3] * hit.t;
+            lod = {cw, st.d};
+        }
+        bool flip = !hit.front && double_sided;
+        Mat mat = material(S, hit.slot, hit.b, surf.sn, lod, flip);
+        if (flip) {
+            mat.normal = -mat.normal;
+            mat.cc_normal = -mat.cc_normal;
+        }
+        if ((hit.front || double_sided) && !is_black(mat.emission)) {
+            F3 le = mat.emission;
+            if (st.k == 0) {
+                st.radiance = st.radiance + st.beta * le;
+            } else {
+                uint32_t li = tri_light(S, hit.slot);
+                float pl = li == NONE ? 0.0f : tri_pdf(S, li, hit.slot, st.prev_pos, surf.position);
+                float w = 1.0f;
+                if (pl > 0.0f) {
+                    if (strategy == 0) w = power_heuristic(st.prev_pdf, pl);
+                    else if (strategy == 1) w = 0.0f;
+                }
+                if (w > 0.0f) st.radiance = st.radiance + clamp_c(clampv, st.beta * le * w);
+            }
+        }
+        if (mat.unlit) {
+            F3 contrib = st.beta * xyz(mat.base);
+            st.radiance = st.radiance + (st.k == 0 ? contrib : clamp_c(clampv, contrib));
+            return true;
+        }
+        if (st.k >= P.dims[3]) return true;
+        st.bsdf = make_bsdf(S.lds_sheen, mat, -st.d, surf.ng, hit.front);
+        if (bsdf_black(st.bsdf)) return true;
+        st.p = surf.position;
+        st.ng = st.bsdf.ng;
+        st.thickness = mat.thickness;
+        st.transmission = mat.transmission;
+        st.atten_color = mat.atten_color;
+        st.atten_dist = mat.atten_dist;
+        st.front = hit.front;
+        st.pl_u = pattern(lds_sobol, st.pixel_seed, st.index, 2u + 2u * st.k);
+        st.job = 0;
+    }
+
+    // ---- Next NEE shadow ray: punctual j < n, then the area set (wgsl:1863-1997).
+    const uint32_t ray_id = st.k << 16u;
+    while (st.job <= n_punct) {
+        bool ok = false;
+        F3 l = f3(0.0f);
+        float tmax = INF;
+        uint32_t rid = ray_id | 0xffffu;
+        uint32_t skip = NONE;
+        F3 le = f3(0.0f);
+        float pl = -1.0f;
+        if (st.job < n_punct) {
+            LightSample ls = punctual_sample(S, st.job, st.p);
+            if (ls.ok) {
+                ok = true;
+                l = ls.l;
+                if (ls.distance >= 0.0f) tmax = ls.distance * (1.0f - 1.0e-4f);
+                rid = ray_id | 0x8000u | (st.job & 0x7fffu);
+                le = ls.radiance;
+            }
+        } else if (strategy != 2 && (has_env || n_elights > 0)) {
+            float pe = S.p_env;
+            if (st.pl_u.z < pe) {
+                if (has_env) {
+                    EnvSample es = env_sample(S, st.pl_u.x, st.pl_u.y);
+                    if (es.ok && pe * es.pdf > 0.0f) {
+                        ok = true;
+                        l = es.d;
+                        le = es.le;
+                        pl = pe * es.pdf;
+                    }
+                }
+            } else if (n_elights > 0) {
+                float us = st.pl_u.z;
+                if (pe > 0.0f) us = clampf((st.pl_u.z - pe) / (1.0f - pe), 0.0f, 0.999999f);
+                uint32_t li = pick_elight(S, us);
+                uint32_t lslot = elight_slot(S, li);
+                F3 a = tri_pos(S, lslot, 0), b = tri_pos(S, lslot, 1), c = tri_pos(S, lslot, 2);
+                bool has_target = false;
+                F3 q = f3(0.0f);
+                if (triangle_solid_angle(st.p, a, b, c) >= MIN_SPHERICAL_SOLID_ANGLE) {
+                    F3 sl = sample_spherical_triangle(st.p, a, b, c, st.pl_u.x, st.pl_u.y);
+                    if (dot(sl, sl) > 0.0f) {
+                        F3 n = cross(b - a, c - a);
+                        float dn = dot(sl, n);
+                        if (dn != 0.0f) {
+                            float t = dot(a - st.p, n) / dn;
+                            if (t > 0.0f && finite(t)) {
+                                q = st.p + sl * t;
+                                has_target = true;
+                            }
+                        }
+                    }
+                } else {
+                    float su = sqrtf(st.pl_u.x);
+                    F3 w3 = {1.0f - su, su * (1.0f - st.pl_u.y), su * st.pl_u.y};
+                    q = a * w3.x + b * w3.y + c * w3.z;
+                    has_target = true;
+                }
+                if (has_target) {
+                    F3 e1 = b - a, e2 = c - a, rr = q - a;
+                    float d11 = dot(e1, e1), d12 = dot(e1, e2), d22 = dot(e2, e2);
+                    float r1 = dot(rr, e1), r2 = dot(rr, e2);
+                    float det = d11 * d22 - d12 * d12;
+                    F3 to = q - st.p;
+                    float dist2 = dot(to, to);
+                    if (finite(det) && fabsf(det) > 1.17549435e-38f && dist2 > 1.0e-12f) {
+                        float bu = (d22 * r1 - d12 * r2) / det;
+                        float bv = (d11 * r2 - d12 * r1) / det;
+                        F3 lb = vmin(vmax(F3{1.0f - bu - bv, bu, bv}, f3(0.0f)), f3(1.0f));
+                        F3 lng = surface(S, lslot, lb).ng;
+                        bool lds = (mat_flags(S, tri_material(S, lslot)) & 4u) != 0;
+                        float dist = sqrtf(dist2);
+                        F3 ld = to / dist;
+                        float cos_l = -dot(lng, ld);
+                        float tp = tri_pdf(S, li, lslot, st.p, q);
+                        if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && tp > 0.0f && finite(tp)) {
+                            ok = true;
+                            l = ld;
+                            le = emission(S, lslot, lb);
+                            pl = tp;
+                            tmax = fmaxf(dist * (1.0f - 1.0e-4f), 0.0f);
+                            skip = tri_global(S, lslot);
+                        }
+                    }
+                }
+            }
+        }
+        if (ok) {
+            Eval e = bsdf_eval(S.lds_sheen, st.bsdf, l);
+            if (!is_black(e.f) && !is_black(le)) {
+                if (pl < 0.0f) {
+                    st.pending = clamp_c(clampv, st.beta * e.f * le);
+                } else {
+                    float w = 1.0f;
+                    if (strategy == 0) w = power_heuristic(pl, e.pdf);
+                    st.pending = clamp_c(clampv, st.beta * e.f * le * (w / pl));
+                }
+                st.r_o = spawn(st.p, st.ng, l);
+                st.r_d = l;
+                st.r_tmax = tmax;
+                st.r_id = rid;
+                st.r_skip = skip;
+                st.shadow = true;
+                return false;
+            }
+        }
+        st.job += 1;
+    }
+
+    // ---- BSDF sampling -> next path ray (wgsl:1999-2039).
+    float4 pb_u = pattern(lds_sobol, st.pixel_seed, st.index, 1u + 2u * st.k);
+    BSample bs = bsdf_sample(S.lds_sheen, st.bsdf, F3{pb_u.x, pb_u.y, pb_u.z});
+    if (!bs.ok) return true;
+    st.beta = st.beta * (bs.f * (1.0f / bs.pdf));
+    if (!finite3(st.beta)) return true;
+    st.prev_pdf = bs.pdf;
+    st.prev_pos = st.p;
+    if (dot(bs.l, st.ng) < 0.0f && st.thickness > 0.0f && st.transmission > 0.0f) {
+        if (st.front && st.atten_dist >= 0.0f) {
+            F3 lc = vmax(st.atten_color, f3(1.0e-6f));
+            st.sigma = F3{-logf(lc.x), -logf(lc.y), -logf(lc.z)} / st.atten_dist;
+            st.in_medium = true;
+        } else {
+            st.in_medium = false;
+        }
+    }
+    if (st.k + 1u >= P.cfg[0]) {
+        float q = clampf(max3(st.beta), 0.05f, 1.0f);
+        if (q < 1.0f) {
+            if (pb_u.w >= q) return true;
+            st.beta = st.beta * (1.0f / q);
+        }
+    }
+    st.k += 1;
+    st.d = bs.l;
+    st.shadow = false;
+    st.r_o = spawn(st.p, st.ng, bs.l);
+    st.r_d = st.d;
+    st.r_tmax = INF;
+    st.r_id = st.k << 16u;
+    st.r_skip = NONE;
+    return false;
+}
+
+__device__ __forceinline__ void stage_tables(const TraceArgs &A, uint32_t *lds_sobol, float *lds_sheen) {
+    for (uint32_t i = threadIdx.x; i < OXPT_SOBOL_WORDS; i += blockDim.x) lds_sobol[i] = A.tables[A.P.offs0[0] + i];
+    for (uint32_t i = threadIdx.x; i < OXPT_SHEEN_WORDS; i += blockDim.x) lds_sheen[i] = bitsf(A.tables[A.P.offs0[1] + i]);
+    __syncthreads();
+}
+
+// Work item -> pixel: items run through 8x8 tiles, tiles row-major.
+__device__ __forceinline__ void item_pixel(const TraceArgs &A, uint32_t item, uint32_t &x, uint32_t &y) {
+    uint32_t tile = item >> 6, in = item & 63u;
+    uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
+    x = tx * 8u + (in & 7u);
+    y = ty * 8u + (in >> 3);
+}
+
+__global__ void __launch_bounds__(256) trace_persistent(TraceArgs A) {
+    __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
+    __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
+    stage_tables(A, lds_sobol, lds_sheen);
+    const Scene S = make_scene(A, lds_sheen);
+    const uint32_t lane = __lane_id();
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+    const uint32_t w = A.P.dims[0], h = A.P.dims[1];
+    bool has_pixel = false, exhausted = false, need_init = false;
+    uint32_t px = 0, py = 0, done = 0;
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    PathState st;
+    for (;;) {
+        // Regenerate: idle lanes claim pixels with one atomic per wave.
+        bool want = !has_pixel && !exhausted;
+        uint64_t m = __ballot(want);
+        if (m) {
+            uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (want) {
+                for (;;) {
+                    uint32_t item = base + (uint32_t)__popcll(m & lt_mask);
+                    if (item >= A.n_items) {
+                        exhausted = true;
+                        break;
+                    }
+                    item_pixel(A, item, px, py);
+                    if (px < w && py < h) {
+                        has_pixel = true;
+                        done = 0;
+                        need_init = true;
+                        acc = A.accum[(size_t)py * w + px];
+                        break;
+                    }
+                    // Item outside the image (ragged edge tiles): no pixel,
+                    // try again on the next ballot.
+                    break;
+                }
+            }
+        }
+        if (__ballot(has_pixel || (!exhausted)) == 0) break;
+        if (!has_pixel) continue;
+        if (need_init) {
+            init_path(A, lds_sobol, px, py, A.first + done, st);
+            need_init = false;
+        }
+        bool covered;
+        if (step_path(A, S, lds_sobol, st, &covered)) {
+            if (covered) {
+                F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
+                acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
+            }
+            done += 1;
+            if (done == A.count) {
+                A.accum[(size_t)py * w + px] = acc;
+                has_pixel = false;
+            } else {
+                need_init = true;
+            }
+        }
+    }
+}
+
+// Plain form: one thread per pixel of an 8x8 tile, samples in order.
+__global__ void __launch_bounds__(64) trace_tiles(TraceArgs A) {
+    __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
+    __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
+    stage_tables(A, lds_sobol, lds_sheen);
+    const Scene S = make_scene(A, lds_sheen);
+    uint32_t px, py;
+    item_pixel(A, blockIdx.x * 64u + threadIdx.x, px, py);
+    const uint32_t w = A.P.dims[0], h = A.P.dims[1];
+    if (px >= w || py >= h) return;
+    float4 acc = A.accum[(size_t)py * w + px];
+    PathState st;
+    for (uint32_t i = 0; i < A.count; i++) {
+        init_path(A, lds_sobol, px, py, A.first + i, st);
+        bool covered;
+        while (!step_path(A, S, lds_sobol, st, &covered)) {
+        }
+        if (covered) {
+            F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
+            acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
+        }
+    }
+    A.accum[(size_t)py * w + px] = acc;
+}
+
+__global__ void reset_counter(uint32_t *c) {
+    if (threadIdx.x == 0) *c = 0;
+}
+
+// ---- Resolve (resolve.wgsl:38-75,151-188) ------------------------------
+
+__device__ __forceinline__ float srgb_to_linear(float c) {
+    if (c <= 0.04045f) return c / 12.92f;
+    return powf((c + 0.055f) / 1.055f, 2.4f);
+}
+__device__ __forceinline__ float linear_to_srgb(float c) {
+    float x = clampf(c, 0.0f, 1.0f);
+    if (x <= 0.0031308f) return 12.92f * x;
+    return 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+}
+__device__ __forceinline__ float aces(float x) {
+    float y = x * 0.6f;
+    return clampf((y * (2.51f * y + 0.03f)) / (y * (2.43f * y + 0.59f) + 0.14f), 0.0f, 1.0f);
+}
+__device__ __forceinline__ F3 tone_map(uint32_t op, F3 rgb_in) {
+    F3 rgb = vmax(rgb_in, f3(0.0f));
+    if (op == 1) {
+        float l = dot(rgb, F3{0.2126f, 0.7152f, 0.0722f});
+        if (l <= 0.0f) return f3(0.0f);
+        return vmin(rgb / (1.0f + l), f3(1.0f));
+    }
+    if (op == 2) return {aces(rgb.x), aces(rgb.y), aces(rgb.z)};
+    return vmin(rgb, f3(1.0f));
+}
+__device__ __forceinline__ uint32_t unorm8(float v) { return (uint32_t)floorf(clampf(v, 0.0f, 1.0f) * 255.0f + 0.5f); }
+
+__global__ void resolve_pt(const float4 *__restrict__ accum, uint32_t n, oxpt_resolve_params rp, void *out) {
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    bool hdr = rp.cfg[3] == 1;
+    F3 bg_lin = {srgb_to_linear(rp.background[0]), srgb_to_linear(rp.background[1]), srgb_to_linear(rp.background[2])};
+    float ba = rp.background[3];
+    float ns = rp.exposure[1];
+    float4 a = accum[idx];
+    float4 o;
+    if (a.w <= 0.0f || ns <= 0.0f) {
+        o = make_float4(bg_lin.x, bg_lin.y, bg_lin.z, ba);
+    } else {
+        F3 m = F3{a.x, a.y, a.z} / a.w;
+        float f = clampf(a.w / ns, 0.0f, 1.0f);
+        F3 c = m;
+        if (!hdr) c = tone_map(rp.cfg[2], m * rp.exposure[0]);
+        if (f >= 1.0f) {
+            o = make_float4(c.x, c.y, c.z, 1.0f);
+        } else {
+            float a_sum = f + (1.0f - f) * ba;
+            if (a_sum > 0.0f) {
+                F3 r = (f * c + (1.0f - f) * bg_lin * ba) / a_sum;
+                o = make_float4(r.x, r.y, r.z, a_sum);
+            } else {
+                F3 r = f * c + (1.0f - f) * bg_lin;
+                o = make_float4(r.x, r.y, r.z, 0.0f);
+            }
+        }
+    }
+    if (hdr) {
+        reinterpret_cast<float4 *>(out)[idx] = o;
+    } else {
+        uint32_t px = unorm8(linear_to_srgb(o.x)) | unorm8(linear_to_srgb(o.y)) << 8 |
+                      unorm8(linear_to_srgb(o.z)) << 16 | unorm8(o.w) << 24;
+        reinterpret_cast<uint32_t *>(out)[idx] = px;
+    }
+}
+
+// ---- Parity probes -------------------------------------------------------
+
+__global__ void debug_pattern_words(const uint32_t *__restrict__ tables, uint32_t off_sobol, uint32_t seed, uint32_t n,
+                                    const uint32_t *x, const uint32_t *y, const uint32_t *index, const uint32_t *p,
+                                    uint4 *out) {
+    __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
+    for (uint32_t i = threadIdx.x; i < OXPT_SOBOL_WORDS; i += blockDim.x) lds_sobol[i] = tables[off_sobol + i];
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t pixel_seed = hash_mix(hash_mix(seed, y[i]), x[i]);
+    uint32_t w[4];
+    sobol_owen_words(lds_sobol, index[i], hash_mix(pixel_seed, p[i]), w);
+    out[i] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__global__ void debug_light_picks(TraceArgs A, uint32_t n, const float *u, uint32_t *out3) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Scene S = make_scene(A, nullptr);
+    float v = u[i];
+    out3[3 * i] = S.n_emissive > 0 ? pick_elight(S, v) : NONE;
+    if (A.P.env[2]) {
+        uint32_t yy = env_row(S, v);
+        out3[3 * i + 1] = yy;
+        out3[3 * i + 2] = env_col(S, yy, v);
+    } else {
+        out3[3 * i + 1] = NONE;
+        out3[3 * i + 2] = NONE;
+    }
+}
+
+__global__ void debug_closest_hit(TraceArgs A, uint32_t n, const float *o3, const float *d3, uint32_t *out2) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Scene S = make_scene(A, nullptr);
+    F3 o = {o3[3 * i], o3[3 * i + 1], o3[3 * i + 2]}, d = {d3[3 * i], d3[3 * i + 1], d3[3 * i + 2]};
+    Hit h = traverse(S, make_ray(o, d), INF, 0, 0, false, false, NONE);
+    out2[2 * i] = h.slot;
+    out2[2 * i + 1] = h.slot == NONE ? 0u : fbits(h.t);
+}
+
+}  // namespace oxd
+
+// ---- Host launchers (called by the runtime) ------------------------------
+
+namespace oxd {
+
+hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, hipStream_t s) {
+    if (kernel == 1) {
+        uint32_t blocks = args.n_items / 64u;
+        hipLaunchKernelGGL(trace_tiles, dim3(blocks), dim3(64), 0, s, args);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s, args.work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(trace_persistent, dim3(grid), dim3(block), 0, s, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const float4 *accum, uint32_t n, const oxpt_resolve_params &rp, void *out, hipStream_t s) {
+    uint32_t blocks = (n + 255u) / 256u;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(resolve_pt, dim3(blocks), dim3(256), 0, s, accum, n, rp, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_pattern(const uint32_t *tables, uint32_t off_sobol, uint32_t seed, uint32_t n, const uint32_t *x,
+                                const uint32_t *y, const uint32_t *index, const uint32_t *p, uint32_t *out4,
+                                hipStream_t s) {
+    uint32_t blocks = (n + 255u) / 256u;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_pattern_words, dim3(blocks), dim3(256), 0, s, tables, off_sobol, seed, n, x, y, index, p,
+                       reinterpret_cast<uint4 *>(out4));
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_picks(const TraceArgs &args, uint32_t n, const float *u, uint32_t *out3, hipStream_t s) {
+    uint32_t blocks = (n + 255u) / 256u;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_light_picks, dim3(blocks), dim3(256), 0, s, args, n, u, out3);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_hits(const TraceArgs &args, uint32_t n, const float *o3, const float *d3, uint32_t *out2,
+                             hipStream_t s) {
+    uint32_t blocks = (n + 255u) / 256u;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_closest_hit, dim3(blocks), dim3(256), 0, s, args, n, o3, d3, out2);
+    return hipGetLastError();
+}
+
+}  // namespace oxd

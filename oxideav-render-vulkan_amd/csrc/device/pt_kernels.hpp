@@ -1,0 +1,34 @@
+// Kernel argument block and the launchers the runtime calls.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "oxpt/layout.h"
+
+namespace oxd {
+
+struct TraceArgs {
+    oxpt_params P;
+    const float4 *geom;
+    const float4 *attrs;
+    const uint32_t *tables;
+    const uint16_t *atlas;
+    uint32_t atlas_side;
+    float4 *accum;
+    uint32_t first, count;  // sample run of this launch
+    uint32_t n_items;       // 64 * number of 8x8 tiles
+    uint32_t tiles_x;
+    uint32_t *work;         // work-queue counter (persistent kernel)
+};
+
+hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, hipStream_t s);
+hipError_t launch_resolve(const float4 *accum, uint32_t n, const oxpt_resolve_params &rp, void *out, hipStream_t s);
+hipError_t launch_debug_pattern(const uint32_t *tables, uint32_t off_sobol, uint32_t seed, uint32_t n,
+                                const uint32_t *x, const uint32_t *y, const uint32_t *index, const uint32_t *p,
+                                uint32_t *out4, hipStream_t s);
+hipError_t launch_debug_picks(const TraceArgs &args, uint32_t n, const float *u, uint32_t *out3, hipStream_t s);
+hipError_t launch_debug_hits(const TraceArgs &args, uint32_t n, const float *o3, const float *d3, uint32_t *out2,
+                             hipStream_t s);
+
+}  // namespace oxd

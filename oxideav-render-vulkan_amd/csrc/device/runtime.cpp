@@ -1,0 +1,512 @@
+// Device boundary of liboxpt: contexts, resident scenes, accumulators,
+// refine launches and the resolve readback (include/oxpt/oxpt.h, layer 1).
+// Replaces the reference's wgpu device/queue plumbing
+// (oxideav-render-vulkan src/gpu.rs:984-1117, src/pathtrace.rs:584-844).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "oxpt/oxpt.h"
+#include "pt_kernels.hpp"
+
+namespace {
+
+// Device opening is serialised process-wide (src/gpu.rs:978-992).
+std::mutex g_device_init;
+
+struct EventPair {
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct oxpt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::string summary;
+    uint64_t budget = 0, in_use = 0;
+    uint32_t n_cu = 0;
+    oxpt_tuning tuning{256, 32, 1u << 27, 0};
+    uint32_t *d_work = nullptr;
+    void *d_out = nullptr;
+    size_t d_out_bytes = 0;
+    std::vector<EventPair> events;
+    uint64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+struct oxpt_scene {
+    oxpt_ctx *ctx = nullptr;
+    float4 *geom = nullptr;
+    float4 *attrs = nullptr;
+    uint32_t *tables = nullptr;
+    uint16_t *atlas = nullptr;
+    uint32_t atlas_side = 1, atlas_layers = 1;
+    size_t table_words = 0;
+    size_t bytes = 0;
+};
+
+struct oxpt_accum {
+    oxpt_ctx *ctx = nullptr;
+    float4 *d = nullptr;
+    uint32_t w = 0, h = 0;
+    size_t bytes = 0;
+};
+
+namespace {
+
+int fail(oxpt_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(oxpt_ctx *c, const char *what, hipError_t e) {
+    return fail(c, OXPT_E_BACKEND, std::string("hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+int dalloc(oxpt_ctx *c, size_t bytes, void **p, const char *what) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    if (c->in_use + bytes > c->budget) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "hip: %s needs %zu bytes; the device budget leaves %llu", what, bytes,
+                      (unsigned long long)(c->budget > c->in_use ? c->budget - c->in_use : 0));
+        return fail(c, OXPT_E_BACKEND, buf);
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_fail(c, what, e);
+    }
+    c->in_use += bytes;
+    return OXPT_OK;
+}
+
+void dfree(oxpt_ctx *c, void *p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    if (bytes == 0) bytes = 16;
+    c->in_use -= std::min<uint64_t>(c->in_use, bytes);
+}
+
+int setdev(oxpt_ctx *c) {
+    hipError_t e = hipSetDevice(c->device);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "hipSetDevice", e);
+}
+
+void fold_events(oxpt_ctx *c, size_t keep) {
+    while (c->events.size() > keep) {
+        EventPair ep = c->events.front();
+        c->events.erase(c->events.begin());
+        (void)hipEventSynchronize(ep.b);
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ep.a, ep.b) == hipSuccess) c->total_ms += ms;
+        (void)hipEventDestroy(ep.a);
+        (void)hipEventDestroy(ep.b);
+    }
+}
+
+oxd::TraceArgs trace_args(const oxpt_scene *s, oxpt_accum *acc, const oxpt_params *P) {
+    oxd::TraceArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.P = *P;
+    a.geom = s->geom;
+    a.attrs = s->attrs;
+    a.tables = s->tables;
+    a.atlas = s->atlas;
+    a.atlas_side = s->atlas_side;
+    if (acc) {
+        a.accum = acc->d;
+        uint32_t tx = (acc->w + 7u) / 8u, ty = (acc->h + 7u) / 8u;
+        a.tiles_x = tx;
+        a.n_items = 64u * tx * ty;
+    }
+    a.work = s->ctx->d_work;
+    return a;
+}
+
+// Offsets the kernels dereference must lie inside the uploaded words.
+int check_params(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P) {
+    size_t tw = s->table_words;
+    auto in = [&](uint64_t off, uint64_t n) { return off + n <= tw; };
+    if (!in(P->offs0[0], OXPT_SOBOL_WORDS) || !in(P->offs0[1], OXPT_SHEEN_WORDS))
+        return fail(c, OXPT_E_ARG, "params: Sobol' / sheen offsets outside the tables");
+    if (P->cfg[2] && !in(P->offs0[3], 16ull * P->cfg[2])) return fail(c, OXPT_E_ARG, "params: punctual lights outside the tables");
+    if (P->cfg[3] && !in(P->offs1[0], 4ull * P->cfg[3])) return fail(c, OXPT_E_ARG, "params: emissive lights outside the tables");
+    if (P->env[2]) {
+        uint64_t w = P->env[0], h = P->env[1];
+        if (w == 0 || h == 0 || !in(P->offs1[2], h + 1) || !in(P->offs1[3], h * (w + 1)) || !in(P->offs2[0], w * h) ||
+            !in(P->offs2[1], 3 * w * h))
+            return fail(c, OXPT_E_ARG, "params: environment sections outside the tables");
+    }
+    return OXPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oxpt_ctx_create(int ordinal, oxpt_ctx **out) {
+    if (!out) return OXPT_E_ARG;
+    *out = nullptr;
+    std::lock_guard<std::mutex> lock(g_device_init);
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return OXPT_E_BACKEND;
+    if (ordinal < 0 || ordinal >= n) return OXPT_E_BACKEND;
+    auto *c = new oxpt_ctx();
+    c->device = ordinal;
+    if (hipSetDevice(ordinal) != hipSuccess) {
+        delete c;
+        return OXPT_E_BACKEND;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ordinal) != hipSuccess) {
+        delete c;
+        return OXPT_E_BACKEND;
+    }
+    c->n_cu = (uint32_t)prop.multiProcessorCount;
+    c->budget = (uint64_t)prop.totalGlobalMem;
+    c->summary = std::string(prop.name) + " (" + prop.gcnArchName + ", HIP)";
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return OXPT_E_BACKEND;
+    }
+    void *w = nullptr;
+    if (dalloc(c, 256, &w, "work counter") != OXPT_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return OXPT_E_BACKEND;
+    }
+    c->d_work = static_cast<uint32_t *>(w);
+    *out = c;
+    return OXPT_OK;
+}
+
+void oxpt_ctx_destroy(oxpt_ctx *c) {
+    if (!c) return;
+    setdev(c);
+    (void)hipStreamSynchronize(c->stream);
+    fold_events(c, 0);
+    dfree(c, c->d_work, 256);
+    dfree(c, c->d_out, c->d_out_bytes);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *oxpt_last_error(const oxpt_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *oxpt_ctx_summary(const oxpt_ctx *c) { return c ? c->summary.c_str() : ""; }
+
+int oxpt_ctx_set_budget(oxpt_ctx *c, uint64_t bytes) {
+    if (!c) return OXPT_E_ARG;
+    c->budget = bytes;
+    return OXPT_OK;
+}
+uint64_t oxpt_ctx_bytes_in_use(const oxpt_ctx *c) { return c ? c->in_use : 0; }
+
+int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
+    if (!c || !t) return OXPT_E_ARG;
+    if (t->block) {
+        if (t->block != 64 && t->block != 128 && t->block != 256) return fail(c, OXPT_E_ARG, "tuning: block must be 64, 128 or 256");
+        c->tuning.block = t->block;
+    }
+    if (t->waves_per_cu) c->tuning.waves_per_cu = std::min(t->waves_per_cu, 64u);
+    if (t->max_paths) c->tuning.max_paths = t->max_paths;
+    c->tuning.kernel = t->kernel;
+    return OXPT_OK;
+}
+int oxpt_ctx_get_tuning(const oxpt_ctx *c, oxpt_tuning *t) {
+    if (!c || !t) return OXPT_E_ARG;
+    *t = c->tuning;
+    return OXPT_OK;
+}
+
+int oxpt_scene_upload(oxpt_ctx *c, const oxpt_scene_desc *d, oxpt_scene **out) {
+    if (!c || !d || !out) return OXPT_E_ARG;
+    *out = nullptr;
+    if (!d->geom || !d->attrs || !d->tables || d->geom_vec4 == 0 || d->attrs_vec4 == 0 || d->table_words == 0)
+        return fail(c, OXPT_E_ARG, "scene: geom, attrs and tables are required");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    auto *s = new oxpt_scene();
+    s->ctx = c;
+    s->table_words = d->table_words;
+    s->atlas_side = std::max(d->atlas_side, 1u);
+    s->atlas_layers = std::max(d->atlas_layers, 1u);
+    size_t gb = d->geom_vec4 * 16, ab = d->attrs_vec4 * 16, tb = d->table_words * 4;
+    size_t xb = d->atlas ? (size_t)s->atlas_side * s->atlas_side * s->atlas_layers * 8 : 8;
+    void *p;
+    int rc;
+    if ((rc = dalloc(c, gb, &p, "BVH + triangle buffer"))) goto err;
+    s->geom = (float4 *)p;
+    if ((rc = dalloc(c, ab, &p, "vertex attribute buffer"))) goto err;
+    s->attrs = (float4 *)p;
+    if ((rc = dalloc(c, tb, &p, "scene tables"))) goto err;
+    s->tables = (uint32_t *)p;
+    if ((rc = dalloc(c, xb, &p, "texture atlas"))) goto err;
+    s->atlas = (uint16_t *)p;
+    s->bytes = gb + ab + tb + xb;
+    {
+        hipError_t e;
+        if ((e = hipMemcpyAsync(s->geom, d->geom, gb, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(s->attrs, d->attrs, ab, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(s->tables, d->tables, tb, hipMemcpyHostToDevice, c->stream)) != hipSuccess) {
+            rc = hip_fail(c, "scene upload", e);
+            goto err;
+        }
+        if (d->atlas) {
+            if ((e = hipMemcpyAsync(s->atlas, d->atlas, xb, hipMemcpyHostToDevice, c->stream)) != hipSuccess) {
+                rc = hip_fail(c, "atlas upload", e);
+                goto err;
+            }
+        } else if ((e = hipMemsetAsync(s->atlas, 0, xb, c->stream)) != hipSuccess) {
+            rc = hip_fail(c, "atlas clear", e);
+            goto err;
+        }
+        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+            rc = hip_fail(c, "scene upload", e);
+            goto err;
+        }
+    }
+    *out = s;
+    return OXPT_OK;
+err:
+    oxpt_scene_destroy(s);
+    return rc;
+}
+
+int oxpt_scene_set_tables(oxpt_ctx *c, oxpt_scene *s, const uint32_t *tables, size_t words) {
+    if (!c || !s || !tables || words == 0 || s->ctx != c) return OXPT_E_ARG;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, "tables update", e);
+    if (words != s->table_words) {
+        void *p = nullptr;
+        if (int rc = dalloc(c, words * 4, &p, "scene tables")) return rc;
+        dfree(c, s->tables, s->table_words * 4);
+        s->bytes = s->bytes - s->table_words * 4 + words * 4;
+        s->tables = (uint32_t *)p;
+        s->table_words = words;
+    }
+    e = hipMemcpyAsync(s->tables, tables, words * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "tables update", e);
+}
+
+uint64_t oxpt_scene_bytes(const oxpt_scene *s) { return s ? s->bytes : 0; }
+
+void oxpt_scene_destroy(oxpt_scene *s) {
+    if (!s) return;
+    oxpt_ctx *c = s->ctx;
+    setdev(c);
+    (void)hipStreamSynchronize(c->stream);
+    uint64_t before = c->in_use;
+    (void)hipFree(s->geom);
+    (void)hipFree(s->attrs);
+    (void)hipFree(s->tables);
+    (void)hipFree(s->atlas);
+    c->in_use = before - std::min<uint64_t>(before, s->bytes);
+    delete s;
+}
+
+int oxpt_accum_create(oxpt_ctx *c, uint32_t w, uint32_t h, oxpt_accum **out) {
+    if (!c || !out) return OXPT_E_ARG;
+    *out = nullptr;
+    if (w == 0 || h == 0) return fail(c, OXPT_E_INVALID, "accumulator: zero dimension");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    auto *a = new oxpt_accum();
+    a->ctx = c;
+    a->w = w;
+    a->h = h;
+    a->bytes = (size_t)w * h * 16;
+    void *p;
+    int rc = dalloc(c, a->bytes, &p, "accumulation buffer");
+    if (rc) {
+        delete a;
+        return rc;
+    }
+    a->d = (float4 *)p;
+    hipError_t e = hipMemsetAsync(a->d, 0, a->bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        oxpt_accum_destroy(a);
+        return hip_fail(c, "accumulator clear", e);
+    }
+    *out = a;
+    return OXPT_OK;
+}
+
+void oxpt_accum_destroy(oxpt_accum *a) {
+    if (!a) return;
+    setdev(a->ctx);
+    (void)hipStreamSynchronize(a->ctx->stream);
+    dfree(a->ctx, a->d, a->bytes);
+    delete a;
+}
+
+int oxpt_accum_clear(oxpt_ctx *c, oxpt_accum *a) {
+    if (!c || !a || a->ctx != c) return OXPT_E_ARG;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipError_t e = hipMemsetAsync(a->d, 0, a->bytes, c->stream);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "accumulator clear", e);
+}
+
+int oxpt_accum_read(oxpt_ctx *c, const oxpt_accum *a, float *host, size_t bytes) {
+    if (!c || !a || !host || a->ctx != c) return OXPT_E_ARG;
+    if (bytes < a->bytes) return fail(c, OXPT_E_ARG, "accumulator read: buffer too small");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipError_t e = hipMemcpyAsync(host, a->d, a->bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "accumulator read", e);
+}
+
+int oxpt_accum_write(oxpt_ctx *c, oxpt_accum *a, const float *host, size_t bytes) {
+    if (!c || !a || !host || a->ctx != c) return OXPT_E_ARG;
+    if (bytes < a->bytes) return fail(c, OXPT_E_ARG, "accumulator write: buffer too small");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipError_t e = hipMemcpyAsync(a->d, host, a->bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "accumulator write", e);
+}
+
+int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_params *P, uint32_t first, uint32_t count) {
+    if (!c || !s || !a || !P || s->ctx != c || a->ctx != c) return OXPT_E_ARG;
+    if (P->dims[0] != a->w || P->dims[1] != a->h) return fail(c, OXPT_E_ARG, "refine: params size differs from the accumulator");
+    if (count == 0) return OXPT_OK;
+    if (int rc = check_params(c, s, P)) return rc;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    uint64_t pixels = (uint64_t)a->w * a->h;
+    uint64_t per = std::max<uint64_t>(1, c->tuning.max_paths / std::max<uint64_t>(pixels, 1));
+    uint32_t block = c->tuning.block;
+    uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
+    uint64_t end = (uint64_t)first + count;
+    for (uint64_t f = first; f < end;) {
+        uint32_t n = (uint32_t)std::min<uint64_t>(per, end - f);
+        oxd::TraceArgs args = trace_args(s, a, P);
+        args.first = (uint32_t)f;
+        args.count = n;
+        EventPair ep{};
+        bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
+        if (timed) (void)hipEventRecord(ep.a, c->stream);
+        hipError_t e = oxd::launch_trace(args, c->tuning.kernel, block, grid, c->stream);
+        if (e != hipSuccess) return hip_fail(c, "trace launch", e);
+        if (timed) {
+            (void)hipEventRecord(ep.b, c->stream);
+            c->events.push_back(ep);
+            c->launches++;
+        }
+        if (c->events.size() > 4096) fold_events(c, 2048);
+        f += n;
+    }
+    return OXPT_OK;
+}
+
+int oxpt_finish(oxpt_ctx *c) {
+    if (!c) return OXPT_E_ARG;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "finish", e);
+}
+
+int oxpt_ctx_timing(oxpt_ctx *c, int reset, uint64_t *launches, double *total_ms) {
+    if (!c) return OXPT_E_ARG;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    fold_events(c, 0);
+    if (launches) *launches = c->launches;
+    if (total_ms) *total_ms = c->total_ms;
+    if (reset) {
+        c->launches = 0;
+        c->total_ms = 0.0;
+    }
+    return OXPT_OK;
+}
+
+int oxpt_resolve(oxpt_ctx *c, const oxpt_accum *a, const oxpt_resolve_params *rp, void *host, size_t bytes) {
+    if (!c || !a || !rp || !host || a->ctx != c) return OXPT_E_ARG;
+    size_t need = (size_t)a->w * a->h * (rp->cfg[3] == 1 ? 16 : 4);
+    if (bytes < need) return fail(c, OXPT_E_ARG, "resolve: output buffer too small");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    if (c->d_out_bytes < need) {
+        dfree(c, c->d_out, c->d_out_bytes);
+        c->d_out = nullptr;
+        c->d_out_bytes = 0;
+        if (int rc = dalloc(c, need, &c->d_out, "resolve target")) return rc;
+        c->d_out_bytes = need;
+    }
+    hipError_t e = oxd::launch_resolve(a->d, a->w * a->h, *rp, c->d_out, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, c->d_out, need, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "resolve", e);
+}
+
+int oxpt_debug_pattern_words(oxpt_ctx *c, const oxpt_scene *s, uint32_t seed, size_t n, const uint32_t *x,
+                             const uint32_t *y, const uint32_t *index, const uint32_t *p, uint32_t *out4) {
+    if (!c || !s || !x || !y || !index || !p || !out4 || s->ctx != c) return OXPT_E_ARG;
+    if (n == 0) return OXPT_OK;
+    if (n > (1u << 28)) return fail(c, OXPT_E_ARG, "debug: too many tuples");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    void *d = nullptr;
+    size_t in_b = n * 4, bytes = 4 * in_b + n * 16;
+    if (int rc = dalloc(c, bytes, &d, "debug buffers")) return rc;
+    uint32_t *dx = (uint32_t *)d, *dy = dx + n, *di = dy + n, *dp = di + n, *dout = dp + n;
+    hipError_t e = hipMemcpyAsync(dx, x, in_b, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dy, y, in_b, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(di, index, in_b, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dp, p, in_b, hipMemcpyHostToDevice, c->stream);
+    // The Sobol' table sits at the start of the tables (offset 0) unless the
+    // caller packed it elsewhere; the tracer always packs it first.
+    if (e == hipSuccess) e = oxd::launch_debug_pattern(s->tables, 0, seed, (uint32_t)n, dx, dy, di, dp, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out4, dout, n * 16, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(c, d, bytes);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "debug pattern words", e);
+}
+
+int oxpt_debug_light_picks(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, size_t n, const float *u,
+                           uint32_t *out3) {
+    if (!c || !s || !P || !u || !out3 || s->ctx != c) return OXPT_E_ARG;
+    if (n == 0) return OXPT_OK;
+    if (n > (1u << 28)) return fail(c, OXPT_E_ARG, "debug: too many uniforms");
+    if (int rc = check_params(c, s, P)) return rc;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    void *d = nullptr;
+    size_t bytes = n * 4 + n * 12;
+    if (int rc = dalloc(c, bytes, &d, "debug buffers")) return rc;
+    float *du = (float *)d;
+    uint32_t *dout = (uint32_t *)(du + n);
+    oxd::TraceArgs args = trace_args(s, nullptr, P);
+    hipError_t e = hipMemcpyAsync(du, u, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = oxd::launch_debug_picks(args, (uint32_t)n, du, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out3, dout, n * 12, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(c, d, bytes);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "debug light picks", e);
+}
+
+int oxpt_debug_closest_hit(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, size_t n, const float *o3,
+                           const float *d3, uint32_t *out2) {
+    if (!c || !s || !P || !o3 || !d3 || !out2 || s->ctx != c) return OXPT_E_ARG;
+    if (n == 0) return OXPT_OK;
+    if (n > (1u << 26)) return fail(c, OXPT_E_ARG, "debug: too many rays");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    void *d = nullptr;
+    size_t bytes = n * 12 * 2 + n * 8;
+    if (int rc = dalloc(c, bytes, &d, "debug buffers")) return rc;
+    float *dorig = (float *)d, *ddir = dorig + 3 * n;
+    uint32_t *dout = (uint32_t *)(ddir + 3 * n);
+    oxd::TraceArgs args = trace_args(s, nullptr, P);
+    hipError_t e = hipMemcpyAsync(dorig, o3, n * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ddir, d3, n * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = oxd::launch_debug_hits(args, (uint32_t)n, dorig, ddir, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out2, dout, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(c, d, bytes);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "debug closest hit", e);
+}
+
+}  // extern "C"

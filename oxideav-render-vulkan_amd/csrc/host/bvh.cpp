@@ -1,0 +1,234 @@
+// Binned-SAH BVH over the triangle soup, emitted in the reference's node
+// layout (src/shaders/pathtrace.wgsl:638-735): 32-byte nodes
+// lo = (bmin.xyz, first), hi = (bmax.xyz, count); count > 0 is a leaf over
+// leaf slots [first, first + count), otherwise the children are the
+// adjacent pair (first, first + 1); the root is node 0. Siblings are
+// adjacent so one traversal step reads one contiguous 64-byte pair.
+//
+// The reference takes its tree from oxideav-mesh3d's `Bvh` (not vendored);
+// only the layout is fixed by the kernel. Images do not depend on the
+// topology except through exact-tie order, which the brute-force
+// closest-hit test covers.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "scene.hpp"
+
+namespace oxpt {
+namespace {
+
+constexpr int kBins = 32;
+constexpr uint32_t kMaxLeaf = 4;      // SAH may stop at or below this size
+constexpr uint32_t kHardLeaf = 8;     // never leave more than this in a leaf ...
+constexpr uint32_t kMaxDepth = 56;    // ... unless the depth cap (stack is 64) forces it
+constexpr float kTravCost = 1.0f;
+constexpr float kTriCost = 1.0f;
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const float *p) {
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+    void grow(const Box &b) {
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    float area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0 && dy >= 0 && dz >= 0)) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Node {
+    Box box;
+    uint32_t first = 0, count = 0;
+};
+
+struct Builder {
+    const std::vector<Box> &tb;
+    const std::vector<float> &cen;  // 3 per tri
+    std::vector<uint32_t> &ids;
+    std::vector<Node> nodes;
+    uint32_t depth = 0;
+
+    Builder(const std::vector<Box> &t, const std::vector<float> &c, std::vector<uint32_t> &i)
+        : tb(t), cen(c), ids(i) {}
+
+    // Builds the subtree for ids[b, e) into `nodes[ni]`, children appended.
+    void build(uint32_t ni, uint32_t b, uint32_t e, uint32_t d) {
+        depth = std::max(depth, d);
+        Box box, cb;
+        for (uint32_t i = b; i < e; i++) {
+            box.grow(tb[ids[i]]);
+            cb.grow(&cen[3 * (size_t)ids[i]]);
+        }
+        nodes[ni].box = box;
+        uint32_t n = e - b;
+        if (n <= 1 || d >= kMaxDepth) {
+            nodes[ni].first = b;
+            nodes[ni].count = n;
+            return;
+        }
+        // Binned SAH on every axis with a non-degenerate centroid extent.
+        float best_cost = INFINITY;
+        int best_axis = -1, best_split = 0;
+        for (int ax = 0; ax < 3; ax++) {
+            float ext = cb.hi[ax] - cb.lo[ax];
+            if (!(ext > 0.0f)) continue;
+            float scale = (float)kBins / ext;
+            Box bins[kBins];
+            uint32_t cnt[kBins] = {0};
+            for (uint32_t i = b; i < e; i++) {
+                uint32_t t = ids[i];
+                int bi = (int)((cen[3 * (size_t)t + ax] - cb.lo[ax]) * scale);
+                bi = std::min(std::max(bi, 0), kBins - 1);
+                bins[bi].grow(tb[t]);
+                cnt[bi]++;
+            }
+            float right_area[kBins];
+            uint32_t right_cnt[kBins];
+            Box acc;
+            uint32_t c = 0;
+            for (int i = kBins - 1; i > 0; i--) {
+                acc.grow(bins[i]);
+                c += cnt[i];
+                right_area[i] = acc.area();
+                right_cnt[i] = c;
+            }
+            Box lacc;
+            uint32_t lc = 0;
+            for (int i = 0; i < kBins - 1; i++) {
+                lacc.grow(bins[i]);
+                lc += cnt[i];
+                if (lc == 0 || right_cnt[i + 1] == 0) continue;
+                float cost = lacc.area() * (float)lc + right_area[i + 1] * (float)right_cnt[i + 1];
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = ax;
+                    best_split = i + 1;
+                }
+            }
+        }
+        float parent_area = box.area();
+        float leaf_cost = kTriCost * (float)n;
+        float split_cost = parent_area > 0.0f ? kTravCost + kTriCost * best_cost / parent_area : INFINITY;
+        if (n <= kMaxLeaf && !(split_cost < leaf_cost)) {
+            nodes[ni].first = b;
+            nodes[ni].count = n;
+            return;
+        }
+        uint32_t m;
+        if (best_axis >= 0) {
+            int ax = best_axis;
+            float scale = (float)kBins / (cb.hi[ax] - cb.lo[ax]);
+            float lo = cb.lo[ax];
+            auto mid = std::partition(ids.begin() + b, ids.begin() + e, [&](uint32_t t) {
+                int bi = (int)((cen[3 * (size_t)t + ax] - lo) * scale);
+                bi = std::min(std::max(bi, 0), kBins - 1);
+                return bi < best_split;
+            });
+            m = (uint32_t)(mid - ids.begin());
+        } else {
+            m = b;  // all centroids coincide
+        }
+        if (m == b || m == e) {
+            if (n <= kHardLeaf) {
+                nodes[ni].first = b;
+                nodes[ni].count = n;
+                return;
+            }
+            // Object median along the widest centroid (or box) axis.
+            int ax = 0;
+            float w = -1.0f;
+            for (int k = 0; k < 3; k++) {
+                float ext = cb.hi[k] - cb.lo[k];
+                if (ext > w) {
+                    w = ext;
+                    ax = k;
+                }
+            }
+            m = b + n / 2;
+            std::nth_element(ids.begin() + b, ids.begin() + m, ids.begin() + e, [&](uint32_t x, uint32_t y) {
+                float cx = cen[3 * (size_t)x + ax], cy = cen[3 * (size_t)y + ax];
+                return cx < cy || (cx == cy && x < y);
+            });
+        }
+        uint32_t c = (uint32_t)nodes.size();
+        nodes.emplace_back();
+        nodes.emplace_back();
+        nodes[ni].first = c;
+        nodes[ni].count = 0;
+        build(c, b, m, d + 1);
+        build(c + 1, m, e, d + 1);
+    }
+};
+
+inline uint32_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+}  // namespace
+
+uint32_t build_bvh(const Soup &soup, std::vector<uint32_t> &node_words, std::vector<uint32_t> &order) {
+    size_t n = soup.count();
+    node_words.clear();
+    order.clear();
+    if (n == 0) return 0;
+    std::vector<Box> tb(n);
+    std::vector<float> cen(3 * n);
+    auto prep = [&](size_t b, size_t e) {
+        for (size_t t = b; t < e; t++) {
+            const float *p = &soup.pos[9 * t];
+            Box bx;
+            bx.grow(p);
+            bx.grow(p + 3);
+            bx.grow(p + 6);
+            tb[t] = bx;
+            for (int k = 0; k < 3; k++) cen[3 * t + k] = 0.5f * (bx.lo[k] + bx.hi[k]);
+        }
+    };
+    unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n > 65536 && hw > 1) {
+        std::vector<std::thread> pool;
+        size_t chunk = (n + hw - 1) / hw;
+        for (unsigned i = 0; i < hw; i++) {
+            size_t b = i * chunk, e = std::min(n, b + chunk);
+            if (b < e) pool.emplace_back(prep, b, e);
+        }
+        for (auto &t : pool) t.join();
+    } else {
+        prep(0, n);
+    }
+    order.resize(n);
+    for (size_t i = 0; i < n; i++) order[i] = (uint32_t)i;
+    Builder bld(tb, cen, order);
+    bld.nodes.reserve(2 * n / kMaxLeaf + 8);
+    bld.nodes.emplace_back();
+    bld.build(0, 0, (uint32_t)n, 0);
+    node_words.resize(bld.nodes.size() * 8);
+    for (size_t i = 0; i < bld.nodes.size(); i++) {
+        const Node &nd = bld.nodes[i];
+        uint32_t *w = &node_words[8 * i];
+        for (int k = 0; k < 3; k++) {
+            w[k] = fbits(nd.box.lo[k]);
+            w[4 + k] = fbits(nd.box.hi[k]);
+        }
+        w[3] = nd.first;
+        w[7] = nd.count;
+    }
+    return bld.depth;
+}
+
+}  // namespace oxpt

@@ -1,0 +1,119 @@
+// Host scene model: the triangle soup, materials, textures, lights and
+// camera that the packer turns into the kernels' words. It stands in for
+// the reference's prepared trace scene (oxideav-render `TraceScene`,
+// consumed at oxideav-render-vulkan src/pathtrace.rs:970-1239).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "oxpt/oxpt.h"
+
+namespace oxpt {
+
+struct Texture {
+    uint32_t w = 0, h = 0;
+    std::vector<uint8_t> rgba8;  // level 0
+    oxpt_sampler sampler{2, 2, 1, 5};
+};
+
+// Per-triangle record: positions plus optional per-corner attributes.
+struct Soup {
+    std::vector<float> pos;       // 9 per tri
+    std::vector<float> normals;   // 9 per tri (or empty)
+    std::vector<float> tangents;  // 12 per tri
+    std::vector<float> uv0;       // 6 per tri
+    std::vector<float> uv1;       // 6 per tri
+    std::vector<float> colors;    // 12 per tri
+    std::vector<uint32_t> material;
+    std::vector<uint8_t> flags;   // OXPT_F_* per tri
+    size_t count() const { return material.size(); }
+};
+
+struct HostScene {
+    Soup soup;
+    std::vector<oxpt_material> materials;
+    std::vector<Texture> textures;
+    std::vector<oxpt_light> lights;
+    bool has_camera = false;
+    oxpt_camera camera{};
+    // Content generation: bumped on every edit so cached packs notice.
+    uint64_t generation = 0;
+};
+
+struct EnvMap {
+    uint32_t w = 0, h = 0;
+    std::vector<float> rgb;  // w*h*3
+    float intensity = 1.0f;
+};
+
+// Resolved camera (Camera::resolve): orthonormal basis + half extents.
+struct CameraFrame {
+    float eye[3], forward[3], side[3], up[3];
+    float half_w, half_h;
+    bool ortho;
+};
+
+// What Camera::resolve needs from an uploaded scene: its camera (if any)
+// and its bounds, for framing scenes without one.
+struct CameraSource {
+    bool has_camera = false;
+    oxpt_camera camera{};
+    bool empty = true;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+};
+CameraSource camera_source(const HostScene &s);
+CameraFrame resolve_camera(const CameraSource &s, uint32_t w, uint32_t h);
+
+// Packed scene: the exact words of include/oxpt/layout.h.
+struct Packed {
+    std::vector<float> geom;      // float4 words
+    std::vector<float> attrs;     // float4 words
+    std::vector<uint32_t> tables; // scene part (no environment)
+    std::vector<uint16_t> atlas;  // binary16 RGBA
+    uint32_t atlas_side = 1, atlas_layers = 1;
+    uint32_t node_count = 0, tri_base = 0;
+    uint32_t n_punctual = 0, n_emissive = 0;
+    uint32_t off_sobol = 0, off_sheen = 0, off_mat = 0, off_plights = 0, off_elights = 0;
+    uint32_t bvh_depth = 0;
+};
+
+struct EnvWords {
+    std::vector<uint32_t> words;
+    uint32_t w = 0, h = 0, marg = 0, cond = 0, mass = 0, rad = 0;
+};
+
+// Build the BVH (binned SAH, 32-byte nodes) over the soup: node words (8
+// per node) and the leaf-ordered triangle ids. Returns the tree depth.
+uint32_t build_bvh(const Soup &soup, std::vector<uint32_t> &node_words, std::vector<uint32_t> &order);
+
+// Pack everything but the environment (GpuPathTracer::upload).
+Packed pack_scene(const HostScene &s);
+// Environment section (env_words, src/pathtrace.rs:1428-1461).
+EnvWords env_words(const EnvMap &e);
+
+// Tables with the environment appended and the 176-byte Params
+// (ensure_binding, src/pathtrace.rs:847-926).
+void bind(const Packed &pk, const EnvMap *env, const CameraFrame &cam, const oxpt_render_options &o, uint32_t w,
+          uint32_t h, std::vector<uint32_t> &tables, oxpt_params &P);
+
+void sobol_directions(uint32_t out[OXPT_SOBOL_WORDS]);
+void sheen_table(float out[OXPT_SHEEN_WORDS]);
+uint16_t f32_to_f16(float v);
+
+}  // namespace oxpt
+
+struct oxpt_host_scene {
+    oxpt::HostScene s;
+};
+struct oxpt_env_map {
+    oxpt::EnvMap e;
+};
+// A packed scene on the host (no device): words + params.
+struct oxpt_packed {
+    oxpt::Packed pk;
+    std::vector<uint32_t> tables;
+    oxpt_params P;
+};

@@ -1,0 +1,632 @@
+"""Python mirror of the path tracer's API over the C ABI of ``liboxpt.so``.
+
+The compute path is the HIP library; this module only binds it with
+``ctypes`` so tests and the benchmark read like the reference's own
+(oxideav-render-vulkan ``tests/pathtrace.rs``, ``examples/pt_bench.rs``):
+
+* :class:`GpuPathTracer` — ``sync`` / ``refine`` / ``finish`` / ``image`` /
+  ``hdr`` / ``reset`` / ``samples`` / ``is_converged`` / ``size`` /
+  ``invalidate_scene`` / ``set_environment`` / ``render`` / ``render_hdr``
+  (src/pathtrace.rs:309-725, 1241-1264);
+* :data:`PATHTRACE_BACKEND_NAME` and :func:`register_into` (src/lib.rs:117,329-338);
+* :class:`BackendError` / :class:`InvalidOptions` for the reference's
+  ``Error::Backend`` / ``Error::InvalidOptions``.
+
+Loading fails loudly when the library has not been built: there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Callable, Dict, Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "liboxpt.so")
+
+PATHTRACE_BACKEND_NAME = "gpu-pathtrace"
+
+OK, E_BACKEND, E_INVALID, E_ARG = 0, -1, -2, -3
+
+
+class Error(Exception):
+    """Base error (the reference's ``oxideav_render::Error``)."""
+
+
+class BackendError(Error):
+    """``Error::Backend``: no device, limits, allocation or launch failure."""
+
+
+class InvalidOptions(Error):
+    """``Error::InvalidOptions``: e.g. dimensions beyond the device limit."""
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+u32, i32, f32, u64, sz = C.c_uint32, C.c_int32, C.c_float, C.c_uint64, C.c_size_t
+P = C.POINTER
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("eye", f32 * 4), ("forward", f32 * 4), ("side", f32 * 4), ("up", f32 * 4),
+        ("dims", u32 * 4), ("cfg", u32 * 4), ("env", u32 * 4),
+        ("offs0", u32 * 4), ("offs1", u32 * 4), ("offs2", u32 * 4), ("fparams", f32 * 4),
+    ]
+
+
+class Frame(C.Structure):
+    _fields_ = [("first", u32), ("count", u32), ("y0", u32), ("rows", u32)]
+
+
+class ResolveParams(C.Structure):
+    _fields_ = [("cfg", u32 * 4), ("exposure", f32 * 4), ("background", f32 * 4)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("geom", P(f32)), ("geom_vec4", sz), ("attrs", P(f32)), ("attrs_vec4", sz),
+        ("tables", P(u32)), ("table_words", sz), ("atlas", P(C.c_uint16)),
+        ("atlas_side", u32), ("atlas_layers", u32),
+    ]
+
+
+class TexBinding(C.Structure):
+    _fields_ = [("texture", i32), ("uv_set", u32), ("xform", f32 * 6)]
+
+
+class Material(C.Structure):
+    _fields_ = [
+        ("base_color", f32 * 4), ("emissive", f32 * 3), ("alpha_mode", u32), ("alpha_cutoff", f32),
+        ("double_sided", u32), ("unlit", u32),
+        ("metallic", f32), ("roughness", f32), ("normal_scale", f32), ("ior", f32),
+        ("has_specular", u32), ("specular", f32), ("specular_color", f32 * 3),
+        ("has_transmission", u32), ("transmission", f32),
+        ("has_volume", u32), ("thickness", f32), ("attenuation_color", f32 * 3), ("attenuation_distance", f32),
+        ("has_clearcoat", u32), ("clearcoat", f32), ("clearcoat_roughness", f32), ("clearcoat_normal_scale", f32),
+        ("has_sheen", u32), ("sheen_color", f32 * 3), ("sheen_roughness", f32),
+        ("tex", TexBinding * 13),
+    ]
+
+
+class Sampler(C.Structure):
+    _fields_ = [("wrap_s", u32), ("wrap_t", u32), ("mag", u32), ("min", u32)]
+
+
+class Light(C.Structure):
+    _fields_ = [
+        ("kind", u32), ("position", f32 * 3), ("direction", f32 * 3), ("color", f32 * 3),
+        ("intensity", f32), ("range", f32), ("inner_cone", f32), ("outer_cone", f32),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("eye", f32 * 3), ("target", f32 * 3), ("up", f32 * 3), ("yfov", f32),
+        ("orthographic", u32), ("ymag", f32),
+    ]
+
+
+class COptions(C.Structure):
+    _fields_ = [
+        ("width", u32), ("height", u32), ("samples_per_pixel", u32), ("max_bounces", u32),
+        ("rr_start", u32), ("strategy", u32), ("seed", u32), ("clamp", f32), ("ambient", f32),
+        ("background", C.c_uint8 * 4), ("tone_map", u32), ("exposure", f32),
+    ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block", u32), ("waves_per_cu", u32), ("max_paths", u32), ("kernel", u32)]
+
+
+ABI_STRUCTS = [Params, Frame, ResolveParams, SceneDesc, TexBinding, Material, Sampler, Light, Camera,
+               COptions, Tuning]
+
+_vp = C.c_void_p
+
+
+def _sig(name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+# Layer 1.
+_sig("oxpt_ctx_create", C.c_int, C.c_int, P(_vp))
+_sig("oxpt_ctx_destroy", None, _vp)
+_sig("oxpt_last_error", C.c_char_p, _vp)
+_sig("oxpt_ctx_summary", C.c_char_p, _vp)
+_sig("oxpt_ctx_set_budget", C.c_int, _vp, u64)
+_sig("oxpt_ctx_bytes_in_use", u64, _vp)
+_sig("oxpt_scene_upload", C.c_int, _vp, P(SceneDesc), P(_vp))
+_sig("oxpt_scene_set_tables", C.c_int, _vp, _vp, P(u32), sz)
+_sig("oxpt_scene_destroy", None, _vp)
+_sig("oxpt_scene_bytes", u64, _vp)
+_sig("oxpt_accum_create", C.c_int, _vp, u32, u32, P(_vp))
+_sig("oxpt_accum_destroy", None, _vp)
+_sig("oxpt_accum_clear", C.c_int, _vp, _vp)
+_sig("oxpt_accum_read", C.c_int, _vp, _vp, P(f32), sz)
+_sig("oxpt_accum_write", C.c_int, _vp, _vp, P(f32), sz)
+_sig("oxpt_refine", C.c_int, _vp, _vp, _vp, P(Params), u32, u32)
+_sig("oxpt_finish", C.c_int, _vp)
+_sig("oxpt_resolve", C.c_int, _vp, _vp, P(ResolveParams), _vp, sz)
+_sig("oxpt_ctx_timing", C.c_int, _vp, C.c_int, P(u64), P(C.c_double))
+_sig("oxpt_ctx_set_tuning", C.c_int, _vp, P(Tuning))
+_sig("oxpt_ctx_get_tuning", C.c_int, _vp, P(Tuning))
+_sig("oxpt_debug_pattern_words", C.c_int, _vp, _vp, u32, sz, P(u32), P(u32), P(u32), P(u32), P(u32))
+_sig("oxpt_debug_light_picks", C.c_int, _vp, _vp, P(Params), sz, P(f32), P(u32))
+_sig("oxpt_debug_closest_hit", C.c_int, _vp, _vp, P(Params), sz, P(f32), P(f32), P(u32))
+# Layer 2.
+_sig("oxpt_render_options_default", None, P(COptions))
+_sig("oxpt_material_default", None, P(Material))
+_sig("oxpt_host_scene_new", _vp)
+_sig("oxpt_host_scene_free", None, _vp)
+_sig("oxpt_host_scene_add_material", C.c_int, _vp, P(Material))
+_sig("oxpt_host_scene_add_texture", C.c_int, _vp, u32, u32, P(C.c_uint8), P(Sampler))
+_sig("oxpt_host_scene_add_triangles", C.c_int, _vp, sz, P(f32), P(f32), P(f32), P(f32), P(f32), P(f32), P(u32))
+_sig("oxpt_host_scene_add_light", C.c_int, _vp, P(Light))
+_sig("oxpt_host_scene_set_camera", C.c_int, _vp, P(Camera))
+_sig("oxpt_host_scene_triangle_count", sz, _vp)
+_sig("oxpt_env_new", _vp, u32, u32, P(f32), f32)
+_sig("oxpt_env_free", None, _vp)
+_sig("oxpt_testscene_room", _vp, u32)
+_sig("oxpt_testscene_furnace", _vp, f32, f32)
+_sig("oxpt_testscene_terrain", _vp, u32, u32, u32, u32, u32, u32)
+_sig("oxpt_env_sky", _vp, u32, u32, u32)
+_sig("oxpt_testscene_layered", _vp, u32, u32, u32, u32, u32)
+_sig("oxpt_tracer_new", C.c_int, P(_vp))
+_sig("oxpt_tracer_new_on", C.c_int, C.c_int, P(_vp))
+_sig("oxpt_tracer_free", None, _vp)
+_sig("oxpt_tracer_last_error", C.c_char_p, _vp)
+_sig("oxpt_tracer_adapter_summary", C.c_char_p, _vp)
+_sig("oxpt_tracer_sync", C.c_int, _vp, _vp, P(COptions), P(C.c_int))
+_sig("oxpt_tracer_refine", C.c_int, _vp, u32)
+_sig("oxpt_tracer_finish", C.c_int, _vp)
+_sig("oxpt_tracer_image", C.c_int, _vp, P(C.c_uint8), sz)
+_sig("oxpt_tracer_hdr", C.c_int, _vp, P(f32), sz)
+_sig("oxpt_tracer_reset", None, _vp)
+_sig("oxpt_tracer_samples", u32, _vp)
+_sig("oxpt_tracer_is_converged", C.c_int, _vp)
+_sig("oxpt_tracer_size", None, _vp, P(u32), P(u32))
+_sig("oxpt_tracer_invalidate_scene", None, _vp)
+_sig("oxpt_tracer_set_environment", C.c_int, _vp, _vp)
+_sig("oxpt_tracer_render", C.c_int, _vp, _vp, P(COptions), P(C.c_uint8), sz)
+_sig("oxpt_tracer_render_hdr", C.c_int, _vp, _vp, P(COptions), P(f32), sz)
+_sig("oxpt_tracer_ctx", _vp, _vp)
+_sig("oxpt_tracer_device_scene", _vp, _vp)
+_sig("oxpt_tracer_params", C.c_int, _vp, P(Params))
+_sig("oxpt_tracer_words", C.c_int, _vp, C.c_int, P(_vp), P(sz))
+_sig("oxpt_tracer_atlas_shape", C.c_int, _vp, P(u32), P(u32))
+_sig("oxpt_pack", C.c_int, _vp, _vp, P(COptions), P(_vp))
+_sig("oxpt_packed_free", None, _vp)
+_sig("oxpt_packed_words", C.c_int, _vp, C.c_int, P(_vp), P(sz))
+_sig("oxpt_packed_params", C.c_int, _vp, P(Params))
+_sig("oxpt_packed_info", C.c_int, _vp, P(u32), P(u32), P(u32))
+_sig("oxpt_registry_make", C.c_int, C.c_char_p, P(_vp))
+_sig("oxpt_abi_sizes", None, P(u32))
+_sig("oxpt_f32_to_f16", C.c_uint16, f32)
+_sig("oxpt_sobol_directions", None, P(u32))
+_sig("oxpt_sheen_table", None, P(f32))
+
+
+def abi_sizes() -> list:
+    out = (u32 * 11)()
+    lib.oxpt_abi_sizes(out)
+    return list(out)
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(P(ct))
+
+
+def _raise(code: int, msg: str):
+    if code == E_INVALID:
+        raise InvalidOptions(msg)
+    if code == E_BACKEND:
+        raise BackendError(msg)
+    raise Error(f"{msg} (status {code})")
+
+
+STRATEGIES = {"mis": 0, "light": 1, "bsdf": 2}
+TONE_MAPS = {"clamp": 0, "reinhard": 1, "aces": 2}
+
+
+@dataclass
+class RenderOptions:
+    """RenderOptions + PathTraceOptions fields the tracer reads."""
+
+    width: int = 256
+    height: int = 256
+    samples_per_pixel: int = 64
+    max_bounces: int = 8
+    rr_start: int = 3
+    strategy: str = "mis"
+    seed: int = 0
+    clamp: float = 0.0
+    ambient: float = 0.0
+    background: Tuple[int, int, int, int] = (16, 16, 20, 255)
+    tone_map: str = "clamp"
+    exposure: float = 1.0
+
+    def to_c(self) -> COptions:
+        o = COptions()
+        lib.oxpt_render_options_default(C.byref(o))
+        o.width, o.height = self.width, self.height
+        o.samples_per_pixel, o.max_bounces, o.rr_start = self.samples_per_pixel, self.max_bounces, self.rr_start
+        o.strategy = STRATEGIES[self.strategy] if isinstance(self.strategy, str) else int(self.strategy)
+        o.seed, o.clamp, o.ambient = self.seed, self.clamp, self.ambient
+        for k in range(4):
+            o.background[k] = self.background[k]
+        o.tone_map = TONE_MAPS[self.tone_map] if isinstance(self.tone_map, str) else int(self.tone_map)
+        o.exposure = self.exposure
+        return o
+
+
+def material(**kw) -> Material:
+    """A glTF material with defaults; keyword fields override."""
+    m = Material()
+    lib.oxpt_material_default(C.byref(m))
+    for k, v in kw.items():
+        cur = getattr(m, k)
+        if isinstance(cur, C.Array):
+            for i, x in enumerate(v):
+                cur[i] = x
+        else:
+            setattr(m, k, v)
+    return m
+
+
+class Scene:
+    """Host scene (triangle soup, materials, textures, lights, camera)."""
+
+    def __init__(self, handle: Optional[int] = None):
+        self.h = handle if handle is not None else lib.oxpt_host_scene_new()
+        if not self.h:
+            raise Error("scene allocation failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.oxpt_host_scene_free(self.h)
+            self.h = None
+
+    @classmethod
+    def room(cls, seed: int = 0) -> "Scene":
+        return cls(lib.oxpt_testscene_room(seed))
+
+    @classmethod
+    def furnace(cls, rho: float, le: float) -> "Scene":
+        return cls(lib.oxpt_testscene_furnace(rho, le))
+
+    @classmethod
+    def terrain(cls, seed=1, gx=1024, gz=512, n_spheres=200, subdiv=3, n_emissive=64) -> "Scene":
+        return cls(lib.oxpt_testscene_terrain(seed, gx, gz, n_spheres, subdiv, n_emissive))
+
+    @classmethod
+    def layered(cls, seed=2, n_tris=4_000_000, n_textures=16, tex_size=2048, n_lights=16) -> "Scene":
+        return cls(lib.oxpt_testscene_layered(seed, n_tris, n_textures, tex_size, n_lights))
+
+    def add_material(self, m: Material) -> int:
+        return lib.oxpt_host_scene_add_material(self.h, C.byref(m))
+
+    def add_texture(self, rgba8: np.ndarray, sampler: Tuple[int, int, int, int] = (2, 2, 1, 5)) -> int:
+        a = np.ascontiguousarray(rgba8, dtype=np.uint8)
+        s = Sampler(*sampler)
+        return lib.oxpt_host_scene_add_texture(self.h, a.shape[1], a.shape[0], _ptr(a, C.c_uint8), C.byref(s))
+
+    def add_triangles(self, pos, material, normals=None, tangents=None, uv0=None, uv1=None, colors=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 9)
+        n = pos.shape[0]
+        mat = np.ascontiguousarray(np.broadcast_to(np.asarray(material, dtype=np.uint32), (n,)))
+
+        def opt(a, per):
+            if a is None:
+                return None, None
+            arr = np.ascontiguousarray(a, dtype=np.float32).reshape(n, per)
+            return arr, _ptr(arr, f32)
+
+        keep = [opt(normals, 9), opt(tangents, 12), opt(uv0, 6), opt(uv1, 6), opt(colors, 12)]
+        rc = lib.oxpt_host_scene_add_triangles(self.h, n, _ptr(pos, f32), *[k[1] for k in keep], _ptr(mat, u32))
+        if rc:
+            _raise(rc, "add_triangles")
+
+    def add_light(self, kind="point", position=(0, 0, 0), direction=(0, -1, 0), color=(1, 1, 1), intensity=1.0,
+                  range=0.0, inner_cone=0.0, outer_cone=0.7853982):
+        k = {"directional": 0, "point": 1, "spot": 2}[kind] if isinstance(kind, str) else int(kind)
+        l = Light(k, (f32 * 3)(*position), (f32 * 3)(*direction), (f32 * 3)(*color), intensity, range, inner_cone,
+                  outer_cone)
+        lib.oxpt_host_scene_add_light(self.h, C.byref(l))
+
+    def set_camera(self, eye, target, yfov=0.8, up=(0, 1, 0), orthographic=False, ymag=1.0):
+        c = Camera((f32 * 3)(*eye), (f32 * 3)(*target), (f32 * 3)(*up), yfov, 1 if orthographic else 0, ymag)
+        lib.oxpt_host_scene_set_camera(self.h, C.byref(c))
+
+    def triangle_count(self) -> int:
+        return lib.oxpt_host_scene_triangle_count(self.h)
+
+
+class EnvironmentMap:
+    """Equirectangular HDR environment (EnvironmentMap)."""
+
+    def __init__(self, rgb: Optional[np.ndarray] = None, intensity: float = 1.0, handle=None):
+        if handle is not None:
+            self.h = handle
+        else:
+            a = np.ascontiguousarray(rgb, dtype=np.float32)
+            self.h = lib.oxpt_env_new(a.shape[1], a.shape[0], _ptr(a, f32), intensity)
+        if not self.h:
+            raise Error("environment allocation failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.oxpt_env_free(self.h)
+            self.h = None
+
+    @classmethod
+    def sky(cls, w=1024, h=512, seed=0) -> "EnvironmentMap":
+        return cls(handle=lib.oxpt_env_sky(w, h, seed))
+
+
+class Packed:
+    """Host-only packed words + Params (no device), for the CPU oracle."""
+
+    def __init__(self, scene: Scene, opts: RenderOptions, env: Optional[EnvironmentMap] = None):
+        h = _vp()
+        o = opts.to_c()
+        rc = lib.oxpt_pack(scene.h, env.h if env else None, C.byref(o), C.byref(h))
+        if rc:
+            _raise(rc, "pack")
+        self.h = h.value
+        self.geom = self._words(0, np.float32, 4)
+        self.attrs = self._words(1, np.float32, 4)
+        self.tables = self._words(2, np.uint32, 1)
+        self.atlas = self._words(3, np.uint16, 1)
+        p = Params()
+        lib.oxpt_packed_params(self.h, C.byref(p))
+        self.params = p
+        side, layers, depth = u32(), u32(), u32()
+        lib.oxpt_packed_info(self.h, C.byref(side), C.byref(layers), C.byref(depth))
+        self.atlas_side, self.atlas_layers, self.bvh_depth = side.value, layers.value, depth.value
+
+    def _words(self, which, dt, per):
+        ptr, n = _vp(), sz()
+        lib.oxpt_packed_words(self.h, which, C.byref(ptr), C.byref(n))
+        if n.value == 0 or not ptr.value:
+            return np.zeros(0, dtype=dt)
+        ct = {np.float32: f32, np.uint32: u32, np.uint16: C.c_uint16}[dt]
+        arr = np.ctypeslib.as_array(C.cast(ptr.value, P(ct)), shape=(n.value * per,))
+        return arr.copy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.oxpt_packed_free(self.h)
+            self.h = None
+
+
+class GpuPathTracer:
+    """Progressive GPU path tracer (GpuPathTracer, src/pathtrace.rs:186-725)."""
+
+    def __init__(self, device: int = 0, _handle: Optional[int] = None):
+        if _handle is None:
+            h = _vp()
+            rc = lib.oxpt_tracer_new_on(device, C.byref(h))
+            if rc:
+                _raise(rc, "no HIP device: the GPU path tracer needs a gfx950 device")
+            _handle = h.value
+        self.h = _handle
+        self._opts: Optional[RenderOptions] = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.oxpt_tracer_free(self.h)
+            self.h = None
+
+    def _check(self, rc, what):
+        if rc:
+            _raise(rc, f"{what}: {lib.oxpt_tracer_last_error(self.h).decode()}")
+
+    def adapter_summary(self) -> str:
+        return lib.oxpt_tracer_adapter_summary(self.h).decode()
+
+    def sync(self, scene: Scene, opts: RenderOptions) -> bool:
+        o = opts.to_c()
+        r = C.c_int(0)
+        self._check(lib.oxpt_tracer_sync(self.h, scene.h if scene else None, C.byref(o), C.byref(r)), "sync")
+        self._opts = opts
+        return bool(r.value)
+
+    def refine(self, samples: int) -> None:
+        self._check(lib.oxpt_tracer_refine(self.h, samples), "refine")
+
+    def finish(self) -> None:
+        self._check(lib.oxpt_tracer_finish(self.h), "finish")
+
+    def size(self) -> Tuple[int, int]:
+        w, h = u32(), u32()
+        lib.oxpt_tracer_size(self.h, C.byref(w), C.byref(h))
+        return w.value, h.value
+
+    def image(self) -> np.ndarray:
+        w, h = self.size()
+        w, h = max(w, 1), max(h, 1)
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        self._check(lib.oxpt_tracer_image(self.h, _ptr(out, C.c_uint8), out.nbytes), "image")
+        return out
+
+    def hdr(self) -> np.ndarray:
+        w, h = self.size()
+        w, h = max(w, 1), max(h, 1)
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        self._check(lib.oxpt_tracer_hdr(self.h, _ptr(out, f32), out.nbytes), "hdr")
+        return out
+
+    def reset(self) -> None:
+        lib.oxpt_tracer_reset(self.h)
+
+    def samples(self) -> int:
+        return lib.oxpt_tracer_samples(self.h)
+
+    def is_converged(self) -> bool:
+        return bool(lib.oxpt_tracer_is_converged(self.h))
+
+    def invalidate_scene(self) -> None:
+        lib.oxpt_tracer_invalidate_scene(self.h)
+
+    def set_environment(self, env: Optional[EnvironmentMap]) -> None:
+        self._check(lib.oxpt_tracer_set_environment(self.h, env.h if env else None), "set_environment")
+
+    def render(self, scene: Scene, opts: RenderOptions) -> np.ndarray:
+        o = opts.to_c()
+        w, h = max(opts.width, 1), max(opts.height, 1)
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        self._check(lib.oxpt_tracer_render(self.h, scene.h, C.byref(o), _ptr(out, C.c_uint8), out.nbytes), "render")
+        self._opts = opts
+        return out
+
+    def render_hdr(self, scene: Scene, opts: RenderOptions) -> np.ndarray:
+        o = opts.to_c()
+        w, h = max(opts.width, 1), max(opts.height, 1)
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        self._check(lib.oxpt_tracer_render_hdr(self.h, scene.h, C.byref(o), _ptr(out, f32), out.nbytes),
+                    "render_hdr")
+        self._opts = opts
+        return out
+
+    # ---- device access (timing, tuning, parity probes) -------------------
+    @property
+    def ctx(self) -> int:
+        return lib.oxpt_tracer_ctx(self.h)
+
+    def device_scene(self) -> int:
+        s = lib.oxpt_tracer_device_scene(self.h)
+        if not s:
+            raise Error("device scene: not synced")
+        return s
+
+    def params(self) -> Params:
+        p = Params()
+        self._check(lib.oxpt_tracer_params(self.h, C.byref(p)), "params")
+        return p
+
+    def words(self, which: int) -> np.ndarray:
+        ptr, n = _vp(), sz()
+        self._check(lib.oxpt_tracer_words(self.h, which, C.byref(ptr), C.byref(n)), "words")
+        dt, ct, per = [(np.float32, f32, 4), (np.float32, f32, 4), (np.uint32, u32, 1), (np.uint16, C.c_uint16, 1)][which]
+        if n.value == 0:
+            return np.zeros(0, dtype=dt)
+        return np.ctypeslib.as_array(C.cast(ptr.value, P(ct)), shape=(n.value * per,)).copy()
+
+    def atlas_shape(self) -> Tuple[int, int]:
+        s, l = u32(), u32()
+        self._check(lib.oxpt_tracer_atlas_shape(self.h, C.byref(s), C.byref(l)), "atlas_shape")
+        return s.value, l.value
+
+    def timing(self, reset: bool = False) -> Tuple[int, float]:
+        n, ms = u64(), C.c_double()
+        lib.oxpt_ctx_timing(self.ctx, 1 if reset else 0, C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
+    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel)
+        rc = lib.oxpt_ctx_set_tuning(self.ctx, C.byref(t))
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        lib.oxpt_ctx_get_tuning(self.ctx, C.byref(t))
+        return t
+
+    def debug_pattern_words(self, seed, x, y, index, p) -> np.ndarray:
+        arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (x, y, index, p)]
+        n = arrs[0].size
+        out = np.zeros((n, 4), dtype=np.uint32)
+        rc = lib.oxpt_debug_pattern_words(self.ctx, self.device_scene(), seed, n, *[_ptr(a, u32) for a in arrs],
+                                          _ptr(out, u32))
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        return out
+
+    def debug_light_picks(self, u) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        out = np.zeros((u.size, 3), dtype=np.uint32)
+        p = self.params()
+        rc = lib.oxpt_debug_light_picks(self.ctx, self.device_scene(), C.byref(p), u.size, _ptr(u, f32), _ptr(out, u32))
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        return out
+
+    def debug_closest_hit(self, o, d) -> np.ndarray:
+        o = np.ascontiguousarray(o, dtype=np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(d, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros((o.shape[0], 2), dtype=np.uint32)
+        p = self.params()
+        rc = lib.oxpt_debug_closest_hit(self.ctx, self.device_scene(), C.byref(p), o.shape[0], _ptr(o, f32),
+                                        _ptr(d, f32), _ptr(out, u32))
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        return out
+
+
+class RenderRegistry:
+    """Name -> factory registry (oxideav_render::RenderRegistry)."""
+
+    def __init__(self):
+        self._f: Dict[str, Callable[[], GpuPathTracer]] = {}
+
+    def register(self, name: str, factory: Callable[[], GpuPathTracer]) -> None:
+        self._f[name] = factory
+
+    def make(self, name: str) -> GpuPathTracer:
+        if name not in self._f:
+            raise InvalidOptions(f"no renderer named {name!r}")
+        return self._f[name]()
+
+
+def _make_registered(name: str) -> GpuPathTracer:
+    h = _vp()
+    rc = lib.oxpt_registry_make(name.encode(), C.byref(h))
+    if rc:
+        _raise(rc, f"registry: {name}")
+    return GpuPathTracer(_handle=h.value)
+
+
+def register_into(registry: RenderRegistry) -> None:
+    """Register the path tracer under PATHTRACE_BACKEND_NAME (devices open lazily, per make)."""
+    registry.register(PATHTRACE_BACKEND_NAME, lambda: _make_registered(PATHTRACE_BACKEND_NAME))
+
+
+def f32_to_f16(v: float) -> int:
+    return lib.oxpt_f32_to_f16(v)
+
+
+def sobol_directions() -> np.ndarray:
+    out = np.zeros(128, dtype=np.uint32)
+    lib.oxpt_sobol_directions(_ptr(out, u32))
+    return out
+
+
+def sheen_table() -> np.ndarray:
+    out = np.zeros(512, dtype=np.float32)
+    lib.oxpt_sheen_table(_ptr(out, f32))
+    return out
+
+
+def gpu_available() -> bool:
+    """True when a HIP device can be opened (without keeping it)."""
+    h = _vp()
+    if lib.oxpt_ctx_create(0, C.byref(h)) != 0:
+        return False
+    lib.oxpt_ctx_destroy(h.value)
+    return True

@@ -1,0 +1,140 @@
+"""GPU kernels vs the CPU oracle: bit-exact integers, closest hits, and images.
+
+The integer sampler words and light-CDF indices must match bit for bit
+(they are closed-form integer code, pathtrace.wgsl:158-228,1440-1484); the
+images must reach PSNR >= 60 dB at equal spp, with the reference's
+statistical region checks (tests/pathtrace.rs:88-142) on top.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import oxpt
+from ptutil import oracle_frame, parity, psnr
+
+pytestmark = pytest.mark.gpu
+
+
+def small_terrain():
+    return oxpt.Scene.terrain(seed=3, gx=128, gz=64, n_spheres=24, subdiv=2, n_emissive=64)
+
+
+def test_pattern_words_bit_exact(gpu):
+    scene = oxpt.Scene.room(0)
+    opts = oxpt.RenderOptions(width=64, height=64, samples_per_pixel=1)
+    gpu.sync(scene, opts)
+    gpu.refine(1)
+    rng = np.random.default_rng(7)
+    n = 1 << 20
+    x = rng.integers(0, 4096, n, dtype=np.uint32)
+    y = rng.integers(0, 4096, n, dtype=np.uint32)
+    idx = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    idx[: n // 4] = np.arange(n // 4, dtype=np.uint32)  # low sample indices, the common case
+    p = rng.integers(0, 40, n, dtype=np.uint32)
+    for seed in (0, 7, 0xDEADBEEF):
+        got = gpu.debug_pattern_words(seed, x, y, idx, p)
+        want = oracle.pattern_words(oracle.sobol_directions(), seed, x, y, idx, p)
+        bad = np.nonzero((got != want).any(axis=1))[0]
+        assert bad.size == 0, f"seed {seed}: {bad.size} mismatches, first at tuple {bad[:4]}"
+
+
+def test_light_cdf_indices_bit_exact(gpu):
+    scene = small_terrain()
+    env = oxpt.EnvironmentMap.sky(256, 128, 5)
+    opts = oxpt.RenderOptions(width=32, height=32, samples_per_pixel=1, max_bounces=4)
+    gpu.set_environment(env)
+    gpu.sync(scene, opts)
+    gpu.refine(1)
+    pk = oxpt.Packed(scene, opts, env)
+    osc = oracle.Scene.from_packed(pk)
+    P = gpu.params()
+    assert P.cfg[3] == 64 and P.env[2] == 1
+    rng = np.random.default_rng(11)
+    u = rng.random(1 << 20, dtype=np.float32)
+    # Edges: 0, values just below 1, and every CDF value exactly (ties go right).
+    t = pk.tables
+    cdf = t[P.offs1[0]: P.offs1[0] + 4 * P.cfg[3]: 4].view(np.float32)
+    marg = t[P.offs1[2]: P.offs1[2] + P.env[1] + 1].view(np.float32)
+    edges = np.concatenate([[0.0, 0.999999, np.nextafter(np.float32(1), np.float32(0))], cdf, marg,
+                            np.nextafter(cdf, np.float32(0)), np.nextafter(marg, np.float32(0))]).astype(np.float32)
+    u = np.concatenate([edges, u]).astype(np.float32)
+    got = gpu.debug_light_picks(u)
+    want = osc.light_picks(u)
+    bad = np.nonzero((got != want).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} mismatches; first u={u[bad[:4]]} got {got[bad[:4]]} want {want[bad[:4]]}"
+    gpu.set_environment(None)
+
+
+def test_closest_hit_matches_oracle(gpu):
+    scene = small_terrain()
+    opts = oxpt.RenderOptions(width=32, height=32, samples_per_pixel=1)
+    gpu.set_environment(None)
+    gpu.sync(scene, opts)
+    gpu.refine(1)
+    pk = oxpt.Packed(scene, opts)
+    osc = oracle.Scene.from_packed(pk)
+    rng = np.random.default_rng(3)
+    n = 4096
+    o = np.stack([rng.uniform(-6, 6, n), rng.uniform(2, 8, n), rng.uniform(-3, 3, n)], axis=1).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d[:, 1] = -np.abs(d[:, 1])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    got = gpu.debug_closest_hit(o, d)
+    want = np.array([osc.closest_hit(o[i], d[i]) for i in range(n)], dtype=object)
+    ws = np.array([w[0] for w in want], dtype=np.uint32)
+    wt = np.array([w[1] for w in want], dtype=np.float32).view(np.uint32)
+    wt[ws == 0xFFFFFFFF] = 0
+    assert (got[:, 0] == ws).all(), f"slot mismatches: {(got[:, 0] != ws).sum()}"
+    assert (got[:, 1] == wt).all(), f"t mismatches: {(got[:, 1] != wt).sum()}"
+    assert (ws != 0xFFFFFFFF).sum() > n // 4
+
+
+def test_room_full_frame_psnr(gpu):
+    """Plain small workload: the room at 256x256, 16 spp, depth 8 — full frame vs the oracle."""
+    scene = oxpt.Scene.room(0)
+    opts = oxpt.RenderOptions(width=256, height=256, samples_per_pixel=16, max_bounces=8)
+    gh = gpu.render_hdr(scene, opts)
+    gi = gpu.image()
+    _, ci, ch = oracle_frame(scene, opts)
+    p = parity("room", gh, gi, ch, ci, 60.0)
+    # Against the f64 light sampling of the reference's CPU tracer.
+    _, ci64, ch64 = oracle_frame(scene, opts, flags=oracle.F64_LIGHTS)
+    p64 = parity("room-f64", gh, gi, ch64, ci64, 60.0)
+    print(f"room PSNR {p:.2f} dB (f32 oracle), {p64:.2f} dB (f64-light oracle)")
+
+
+def test_white_furnace(gpu):
+    """tests/pathtrace.rs:295-317: L = Le / (1 - rho) = 2 for every strategy."""
+    scene = oxpt.Scene.furnace(0.5, 1.0)
+    for strategy in ("mis", "light", "bsdf"):
+        opts = oxpt.RenderOptions(width=8, height=8, samples_per_pixel=64, max_bounces=64, strategy=strategy,
+                                  ambient=0.0)
+        img = gpu.render_hdr(scene, opts)
+        m = img[..., :3].mean(axis=(0, 1))
+        assert np.all(np.abs(m - 2.0) < 0.04), f"{strategy}: {m}"
+
+
+def test_terrain_env_psnr(gpu):
+    """MR materials + 64 emissive triangles + sky CDF, full frame vs the oracle."""
+    scene = small_terrain()
+    env = oxpt.EnvironmentMap.sky(256, 128, 5)
+    opts = oxpt.RenderOptions(width=160, height=96, samples_per_pixel=16, max_bounces=12)
+    gpu.set_environment(env)
+    gh = gpu.render_hdr(scene, opts)
+    gi = gpu.image()
+    gpu.set_environment(None)
+    _, ci, ch = oracle_frame(scene, opts, env)
+    p = psnr(gi, ci)
+    print(f"terrain+env PSNR {p:.2f} dB")
+    parity("terrain", gh, gi, ch, ci, 60.0)
+
+
+def test_plain_kernel_matches_persistent(gpu):
+    """Both launch forms add each pixel's samples in order: identical accumulators."""
+    scene = oxpt.Scene.room(0)
+    opts = oxpt.RenderOptions(width=72, height=40, samples_per_pixel=6, max_bounces=8)
+    gpu.set_tuning(kernel=1)
+    a = gpu.render_hdr(scene, opts)
+    gpu.set_tuning(kernel=0)
+    b = gpu.render_hdr(scene, opts)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
