@@ -1748,7 +1748,7 @@ int oxo_render(const oxo_scene *s, const oxpt_params *P, uint32_t first, uint32_
     x1 = std::min(x1, w);
     y1 = std::min(y1, h);
     if (x0 >= x1 || y0 >= y1 || count == 0) return 0;
-    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if (threads <= 0) threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     const uint32_t *sob = s->tables + P->offs0[0];
     const float *lut = reinterpret_cast<const float *>(s->tables + P->offs0[1]);
     std::atomic<uint32_t> next_row{y0};

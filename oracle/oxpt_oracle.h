@@ -80,7 +80,7 @@ void oxo_light_picks(const oxo_scene *s, const oxpt_params *P, size_t n, const f
 
 /* Trace `count` samples from `first` for every pixel of rows [y0, y1) and
  * columns [x0, x1), adding into accum (float4 per pixel, width P->dims[0]).
- * threads <= 0 uses every hardware thread. counts may be NULL. */
+ * threads <= 0 uses up to 16 hardware threads. counts may be NULL. */
 int oxo_render(const oxo_scene *s, const oxpt_params *P, uint32_t first, uint32_t count,
                uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, float *accum, int threads,
                int flags, oxo_counts *counts);
