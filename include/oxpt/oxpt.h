@@ -113,6 +113,25 @@ int oxpt_resolve(oxpt_ctx *ctx, const oxpt_accum *acc, const oxpt_resolve_params
  * stream. Returns launches and total milliseconds since the last reset. */
 int oxpt_ctx_timing(oxpt_ctx *ctx, int reset, uint64_t *launches, double *total_ms);
 
+/* Batched refine (many scenes, cameras and sample runs of uneven size in one
+ * persistent launch): every item adds samples [first, first + count) of
+ * every pixel of its accumulator, in sample order per pixel, so each item's
+ * accumulator equals what oxpt_refine would produce for it alone. Items must
+ * belong to `ctx`; at most one item per accumulator. No counterpart in the
+ * reference (its refine serves one scene at a time). */
+typedef struct oxpt_batch_item {
+    const oxpt_scene *scene;
+    oxpt_accum *accum;
+    oxpt_params params;
+    uint32_t first, count;
+} oxpt_batch_item;
+int oxpt_refine_batch(oxpt_ctx *ctx, const oxpt_batch_item *items, size_t n);
+/* Replay the batched step from a HIP graph captured once per launch shape
+ * (the job table and count are written to device memory before each
+ * replay). Off by default. */
+int oxpt_ctx_set_graph(oxpt_ctx *ctx, int enable);
+uint64_t oxpt_ctx_graph_launches(const oxpt_ctx *ctx);
+
 /* Launch shape and traversal knobs (autotuner). 0 keeps the default. */
 typedef struct oxpt_tuning {
     uint32_t block;          /* threads per workgroup: 64, 128, 256 */
@@ -306,6 +325,51 @@ void oxpt_packed_free(oxpt_packed *p);
 int oxpt_packed_words(const oxpt_packed *p, int which, const void **ptr, size_t *count);
 int oxpt_packed_params(const oxpt_packed *p, oxpt_params *out);
 int oxpt_packed_info(const oxpt_packed *p, uint32_t *atlas_side, uint32_t *atlas_layers, uint32_t *bvh_depth);
+
+/* Serving (continuous batching). A pool of prepared scenes; render requests
+ * name a pool scene and options, refine requests add samples to a finished
+ * render's resident accumulator. Requests are admitted against an HBM
+ * budget (LRU eviction of idle scenes), and every tick one batched launch
+ * advances all active requests by at most spp_per_tick samples; requests
+ * arriving meanwhile join the next tick. Images equal the images rendered
+ * alone (per-pixel sample order is kept). No counterpart in the reference. */
+typedef struct oxpt_server oxpt_server;
+typedef struct oxpt_server_config {
+    uint64_t hbm_budget;    /* bytes for resident scenes + accumulators (default 200 GiB) */
+    uint32_t spp_per_tick;  /* samples per request per batched launch (default 16) */
+    uint32_t max_active;    /* request slots, 0 = unbounded */
+    uint32_t keep_hdr;      /* also keep the RGBA32F result */
+    uint32_t use_graph;     /* replay the batched step from a captured HIP graph */
+} oxpt_server_config;
+typedef struct oxpt_request_info {
+    uint32_t ok, width, height, samples;
+    double latency_s;       /* arrival -> result ready */
+    double queue_s;         /* arrival -> admission */
+    uint64_t paths;         /* paths this request traced */
+} oxpt_request_info;
+typedef struct oxpt_server_stats {
+    uint64_t submitted, completed, rejected, ticks, uploads, evictions, evicted_accums, paths;
+    double busy_s;          /* wall time inside batched launches */
+    uint64_t resident_bytes;
+    uint32_t active, queued;
+} oxpt_server_stats;
+void oxpt_server_config_default(oxpt_server_config *c);
+int oxpt_server_create(int device, const oxpt_server_config *cfg, oxpt_server **out);
+void oxpt_server_destroy(oxpt_server *s);
+oxpt_ctx *oxpt_server_ctx(oxpt_server *s);
+const char *oxpt_server_last_error(const oxpt_server *s);
+/* Pack a scene into the pool (host work, done before serving). */
+int oxpt_server_add_scene(oxpt_server *s, const oxpt_host_scene *scene, const oxpt_env_map *env, uint32_t *id);
+int oxpt_server_submit_render(oxpt_server *s, uint32_t scene_id, const oxpt_render_options *o, uint64_t *req);
+int oxpt_server_submit_refine(oxpt_server *s, uint64_t render_req, uint32_t spp, uint64_t *req);
+/* Block until the request is done (timeout_s < 0: forever); OXPT_E_BACKEND on
+ * timeout or rejection. */
+int oxpt_server_wait(oxpt_server *s, uint64_t req, double timeout_s, oxpt_request_info *info);
+/* Copy a finished request's image: RGBA8 (hdr = 0) or RGBA32F (keep_hdr). */
+int oxpt_server_result(oxpt_server *s, uint64_t req, int hdr, void *out, size_t bytes);
+/* Drop a finished request's host image copies. */
+int oxpt_server_release(oxpt_server *s, uint64_t req);
+int oxpt_server_stats_get(oxpt_server *s, oxpt_server_stats *out);
 
 /* Registry (register_into, src/lib.rs:117,329-338). */
 #define OXPT_PATHTRACE_BACKEND_NAME "gpu-pathtrace"

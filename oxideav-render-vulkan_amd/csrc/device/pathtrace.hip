@@ -22,28 +22,33 @@
 
 namespace oxd {
 
-__device__ __forceinline__ Scene make_scene(const TraceArgs &A, const float *lds_sheen) {
+__device__ __forceinline__ Scene make_scene(const oxpt_params &P, const float4 *geom, const float4 *attrs,
+                                            const uint32_t *tables, const uint16_t *atlas, uint32_t atlas_side,
+                                            const float *lds_sheen) {
     Scene S;
-    S.geom = A.geom;
-    S.attrs = A.attrs;
-    S.tables = A.tables;
-    S.atlas = A.atlas;
-    S.atlas_side = A.atlas_side;
-    S.tri_base = A.P.offs2[2];
-    S.node_count = A.P.env[3];
-    S.off_mat = A.P.offs0[2];
-    S.off_plights = A.P.offs0[3];
-    S.off_elights = A.P.offs1[0];
-    S.n_emissive = A.P.cfg[3];
-    S.env_w = A.P.env[0];
-    S.env_h = A.P.env[1];
-    S.env_marg = A.P.offs1[2];
-    S.env_cond = A.P.offs1[3];
-    S.env_mass = A.P.offs2[0];
-    S.env_rad = A.P.offs2[1];
-    S.p_env = A.P.fparams[2];
+    S.geom = geom;
+    S.attrs = attrs;
+    S.tables = tables;
+    S.atlas = atlas;
+    S.atlas_side = atlas_side;
+    S.tri_base = P.offs2[2];
+    S.node_count = P.env[3];
+    S.off_mat = P.offs0[2];
+    S.off_plights = P.offs0[3];
+    S.off_elights = P.offs1[0];
+    S.n_emissive = P.cfg[3];
+    S.env_w = P.env[0];
+    S.env_h = P.env[1];
+    S.env_marg = P.offs1[2];
+    S.env_cond = P.offs1[3];
+    S.env_mass = P.offs2[0];
+    S.env_rad = P.offs2[1];
+    S.p_env = P.fparams[2];
     S.lds_sheen = lds_sheen;
     return S;
+}
+__device__ __forceinline__ Scene make_scene(const TraceArgs &A, const float *lds_sheen) {
+    return make_scene(A.P, A.geom, A.attrs, A.tables, A.atlas, A.atlas_side, lds_sheen);
 }
 
 __device__ __forceinline__ F3 clamp_c(float cl, F3 c) {
@@ -76,9 +81,8 @@ struct PathState {
 };
 
 // Camera ray of sample `index` of pixel (x, y) (wgsl:1715-1764).
-__device__ __forceinline__ void init_path(const TraceArgs &A, const uint32_t *lds_sobol, uint32_t x, uint32_t y,
+__device__ __forceinline__ void init_path(const oxpt_params &P, const uint32_t *lds_sobol, uint32_t x, uint32_t y,
                                           uint32_t index, PathState &st) {
-    const oxpt_params &P = A.P;
     st.pixel_seed = hash_mix(hash_mix(P.dims[2], y), x);
     st.index = index;
     st.path_seed = hash_mix(st.pixel_seed, index);
@@ -127,8 +131,8 @@ __device__ __forceinline__ void init_path(const TraceArgs &A, const uint32_t *ld
 // One iteration: trace the current ray and advance the state machine
 // (wgsl:1766-2040). Returns true when the path ended; *covered says
 // whether the camera ray hit anything.
-__device__ bool step_path(const TraceArgs &A, const Scene &S, const uint32_t *lds_sobol, PathState &st, bool *covered) {
-    const oxpt_params &P = A.P;
+__device__ bool step_path(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol, PathState &st,
+                          bool *covered) {
     const uint32_t strategy = P.cfg[1];
     const bool has_env = P.env[2] != 0;
     const uint32_t n_elights = P.cfg[3], n_punct = P.cfg[2];
@@ -354,16 +358,17 @@ __device__ bool step_path(const TraceArgs &A, const Scene &S, const uint32_t *ld
     return false;
 }
 
-__device__ __forceinline__ void stage_tables(const TraceArgs &A, uint32_t *lds_sobol, float *lds_sheen) {
-    for (uint32_t i = threadIdx.x; i < OXPT_SOBOL_WORDS; i += blockDim.x) lds_sobol[i] = A.tables[A.P.offs0[0] + i];
-    for (uint32_t i = threadIdx.x; i < OXPT_SHEEN_WORDS; i += blockDim.x) lds_sheen[i] = bitsf(A.tables[A.P.offs0[1] + i]);
+__device__ __forceinline__ void stage_tables(const uint32_t *tables, const oxpt_params &P, uint32_t *lds_sobol,
+                                             float *lds_sheen) {
+    for (uint32_t i = threadIdx.x; i < OXPT_SOBOL_WORDS; i += blockDim.x) lds_sobol[i] = tables[P.offs0[0] + i];
+    for (uint32_t i = threadIdx.x; i < OXPT_SHEEN_WORDS; i += blockDim.x) lds_sheen[i] = bitsf(tables[P.offs0[1] + i]);
     __syncthreads();
 }
 
 // Work item -> pixel: items run through 8x8 tiles, tiles row-major.
-__device__ __forceinline__ void item_pixel(const TraceArgs &A, uint32_t item, uint32_t &x, uint32_t &y) {
+__device__ __forceinline__ void item_pixel(uint32_t tiles_x, uint32_t item, uint32_t &x, uint32_t &y) {
     uint32_t tile = item >> 6, in = item & 63u;
-    uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
+    uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     x = tx * 8u + (in & 7u);
     y = ty * 8u + (in >> 3);
 }
@@ -371,7 +376,7 @@ __device__ __forceinline__ void item_pixel(const TraceArgs &A, uint32_t item, ui
 __global__ void __launch_bounds__(256) trace_persistent(TraceArgs A) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
-    stage_tables(A, lds_sobol, lds_sheen);
+    stage_tables(A.tables, A.P, lds_sobol, lds_sheen);
     const Scene S = make_scene(A, lds_sheen);
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
@@ -396,7 +401,7 @@ __global__ void __launch_bounds__(256) trace_persistent(TraceArgs A) {
                         exhausted = true;
                         break;
                     }
-                    item_pixel(A, item, px, py);
+                    item_pixel(A.tiles_x, item, px, py);
                     if (px < w && py < h) {
                         has_pixel = true;
                         done = 0;
@@ -413,11 +418,11 @@ __global__ void __launch_bounds__(256) trace_persistent(TraceArgs A) {
         if (__ballot(has_pixel || (!exhausted)) == 0) break;
         if (!has_pixel) continue;
         if (need_init) {
-            init_path(A, lds_sobol, px, py, A.first + done, st);
+            init_path(A.P, lds_sobol, px, py, A.first + done, st);
             need_init = false;
         }
         bool covered;
-        if (step_path(A, S, lds_sobol, st, &covered)) {
+        if (step_path(A.P, S, lds_sobol, st, &covered)) {
             if (covered) {
                 F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
                 acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
@@ -437,18 +442,18 @@ __global__ void __launch_bounds__(256) trace_persistent(TraceArgs A) {
 __global__ void __launch_bounds__(64) trace_tiles(TraceArgs A) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
-    stage_tables(A, lds_sobol, lds_sheen);
+    stage_tables(A.tables, A.P, lds_sobol, lds_sheen);
     const Scene S = make_scene(A, lds_sheen);
     uint32_t px, py;
-    item_pixel(A, blockIdx.x * 64u + threadIdx.x, px, py);
+    item_pixel(A.tiles_x, blockIdx.x * 64u + threadIdx.x, px, py);
     const uint32_t w = A.P.dims[0], h = A.P.dims[1];
     if (px >= w || py >= h) return;
     float4 acc = A.accum[(size_t)py * w + px];
     PathState st;
     for (uint32_t i = 0; i < A.count; i++) {
-        init_path(A, lds_sobol, px, py, A.first + i, st);
+        init_path(A.P, lds_sobol, px, py, A.first + i, st);
         bool covered;
-        while (!step_path(A, S, lds_sobol, st, &covered)) {
+        while (!step_path(A.P, S, lds_sobol, st, &covered)) {
         }
         if (covered) {
             F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
@@ -458,8 +463,99 @@ __global__ void __launch_bounds__(64) trace_tiles(TraceArgs A) {
     A.accum[(size_t)py * w + px] = acc;
 }
 
+// Batched form: many (scene, camera, sample run) jobs in one persistent
+// launch. A wave works on one job at a time (so the job's Params and scene
+// pointers stay in scalar registers) and its idle lanes claim that job's
+// pixels with the ballot + one-atomic scheme above; when the job runs dry
+// and the wave's lanes have finished their pixels, the wave moves to the
+// next job that still has work. Jobs are ordered largest first by the
+// host, so the small ones fill the tail. Per-pixel sample order is kept,
+// so every job's image equals its image alone.
+__global__ void __launch_bounds__(256) trace_batch(BatchArgs B) {
+    __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
+    __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
+    // Every job carries the same estimator constants (checked on the host).
+    const uint32_t n_jobs = min(*B.n_jobs, B.max_jobs);
+    if (n_jobs == 0) return;
+    stage_tables(B.jobs[0].tables, B.jobs[0].P, lds_sobol, lds_sheen);
+    const uint32_t lane = __lane_id();
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+    uint32_t j = 0;
+    bool job_live = true;  // wave-uniform: job j may still have unclaimed pixels
+    bool has_pixel = false, need_init = false;
+    uint32_t px = 0, py = 0, done = 0;
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    PathState st;
+    for (;;) {
+        j = __builtin_amdgcn_readfirstlane(j);
+        const Job &J = B.jobs[j];
+        const uint32_t w = J.P.dims[0], h = J.P.dims[1];
+        bool want = !has_pixel && job_live;
+        uint64_t m = __ballot(want);
+        if (m) {
+            uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(B.counters + j, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            bool dry = false;
+            if (want) {
+                uint32_t item = base + (uint32_t)__popcll(m & lt_mask);
+                if (item >= J.n_items) {
+                    dry = true;
+                } else {
+                    item_pixel(J.tiles_x, item, px, py);
+                    if (px < w && py < h) {
+                        has_pixel = true;
+                        done = 0;
+                        need_init = true;
+                        acc = J.accum[(size_t)py * w + px];
+                    }
+                }
+            }
+            if (__ballot(dry)) job_live = false;
+        }
+        if (__ballot(has_pixel) == 0) {
+            if (job_live) continue;  // ragged edge items only: claim again
+            // Next job with unclaimed work.
+            uint32_t nj = j + 1;
+            while (nj < n_jobs && __hip_atomic_load(B.counters + nj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+                                        B.jobs[nj].n_items)
+                nj++;
+            if (nj >= n_jobs) break;
+            j = nj;
+            job_live = true;
+            continue;
+        }
+        if (!has_pixel) continue;
+        const Scene S = make_scene(J.P, J.geom, J.attrs, J.tables, J.atlas, J.atlas_side, lds_sheen);
+        if (need_init) {
+            init_path(J.P, lds_sobol, px, py, J.first + done, st);
+            need_init = false;
+        }
+        bool covered;
+        if (step_path(J.P, S, lds_sobol, st, &covered)) {
+            if (covered) {
+                F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
+                acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
+            }
+            done += 1;
+            if (done == J.count) {
+                J.accum[(size_t)py * w + px] = acc;
+                has_pixel = false;
+            } else {
+                need_init = true;
+            }
+        }
+    }
+}
+
 __global__ void reset_counter(uint32_t *c) {
     if (threadIdx.x == 0) *c = 0;
+}
+
+__global__ void reset_counters(uint32_t *c, const uint32_t *n, uint32_t cap) {
+    uint32_t m = min(*n, cap);
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) c[i] = 0;
 }
 
 // ---- Resolve (resolve.wgsl:38-75,151-188) ------------------------------
@@ -585,6 +681,14 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(trace_persistent, dim3(grid), dim3(block), 0, s, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_batch(const BatchArgs &args, uint32_t block, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, s, args.counters, args.n_jobs, args.max_jobs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(trace_batch, dim3(grid), dim3(block), 0, s, args);
     return hipGetLastError();
 }
 

@@ -22,6 +22,31 @@ struct TraceArgs {
     uint32_t *work;         // work-queue counter (persistent kernel)
 };
 
+// One job of a batched launch: a resident scene, its Params, an accumulator
+// and a sample run. 256 bytes, read with scalar loads by the wave that owns it.
+struct alignas(16) Job {
+    oxpt_params P;  // 176 B
+    const float4 *geom;
+    const float4 *attrs;
+    const uint32_t *tables;
+    const uint16_t *atlas;
+    float4 *accum;
+    uint32_t atlas_side, first, count, tiles_x, n_items;
+    uint32_t pad[5];
+};
+static_assert(sizeof(Job) == 256, "Job is 256 bytes");
+
+// The job count lives in device memory (written with the job table), so
+// the launch arguments never change and the step can be captured once in a
+// HIP graph and replayed.
+struct BatchArgs {
+    const Job *jobs;
+    const uint32_t *n_jobs;
+    uint32_t *counters;  // one work counter per job
+    uint32_t max_jobs;   // capacity of jobs / counters
+};
+
+hipError_t launch_trace_batch(const BatchArgs &args, uint32_t block, uint32_t grid, hipStream_t s);
 hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, hipStream_t s);
 hipError_t launch_resolve(const float4 *accum, uint32_t n, const oxpt_resolve_params &rp, void *out, hipStream_t s);
 hipError_t launch_debug_pattern(const uint32_t *tables, uint32_t off_sobol, uint32_t seed, uint32_t n,

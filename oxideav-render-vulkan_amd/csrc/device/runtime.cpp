@@ -34,6 +34,13 @@ struct oxpt_ctx {
     uint32_t n_cu = 0;
     oxpt_tuning tuning{256, 32, 1u << 27, 0};
     uint32_t *d_work = nullptr;
+    oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[job_cap]
+    uint32_t *d_counters = nullptr;  // job_cap counters + 1 count word
+    size_t job_cap = 0;
+    bool use_graph = false;
+    hipGraphExec_t graph = nullptr;  // captured batched step (reset + trace)
+    uint32_t graph_block = 0, graph_grid = 0;
+    uint64_t graph_launches = 0;
     void *d_out = nullptr;
     size_t d_out_bytes = 0;
     std::vector<EventPair> events;
@@ -43,6 +50,7 @@ struct oxpt_ctx {
 
 struct oxpt_scene {
     oxpt_ctx *ctx = nullptr;
+    std::vector<uint32_t> host_tables;  // estimator constants are checked across batched jobs
     float4 *geom = nullptr;
     float4 *attrs = nullptr;
     uint32_t *tables = nullptr;
@@ -195,6 +203,9 @@ void oxpt_ctx_destroy(oxpt_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     fold_events(c, 0);
     dfree(c, c->d_work, 256);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    dfree(c, c->d_jobs, c->job_cap * sizeof(oxd::Job));
+    dfree(c, c->d_counters, c->job_cap * 4 + 4);
     dfree(c, c->d_out, c->d_out_bytes);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -221,6 +232,14 @@ int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
     c->tuning.kernel = t->kernel;
     return OXPT_OK;
 }
+int oxpt_ctx_set_graph(oxpt_ctx *c, int enable) {
+    if (!c) return OXPT_E_ARG;
+    c->use_graph = enable != 0;
+    return OXPT_OK;
+}
+
+uint64_t oxpt_ctx_graph_launches(const oxpt_ctx *c) { return c ? c->graph_launches : 0; }
+
 int oxpt_ctx_get_tuning(const oxpt_ctx *c, oxpt_tuning *t) {
     if (!c || !t) return OXPT_E_ARG;
     *t = c->tuning;
@@ -251,6 +270,7 @@ int oxpt_scene_upload(oxpt_ctx *c, const oxpt_scene_desc *d, oxpt_scene **out) {
     if ((rc = dalloc(c, xb, &p, "texture atlas"))) goto err;
     s->atlas = (uint16_t *)p;
     s->bytes = gb + ab + tb + xb;
+    s->host_tables.assign(d->tables, d->tables + d->table_words);
     {
         hipError_t e;
         if ((e = hipMemcpyAsync(s->geom, d->geom, gb, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
@@ -295,6 +315,7 @@ int oxpt_scene_set_tables(oxpt_ctx *c, oxpt_scene *s, const uint32_t *tables, si
     }
     e = hipMemcpyAsync(s->tables, tables, words * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) s->host_tables.assign(tables, tables + words);
     return e == hipSuccess ? OXPT_OK : hip_fail(c, "tables update", e);
 }
 
@@ -402,6 +423,141 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         }
         if (c->events.size() > 4096) fold_events(c, 2048);
         f += n;
+    }
+    return OXPT_OK;
+}
+
+int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
+    if (!c || (!items && n)) return OXPT_E_ARG;
+    if (n == 0) return OXPT_OK;
+    // Validate every item; the Sobol' matrices and sheen LUT (staged once
+    // per workgroup) must be the same words in every job.
+    const oxpt_batch_item &i0 = items[0];
+    for (size_t i = 0; i < n; i++) {
+        const oxpt_batch_item &it = items[i];
+        if (!it.scene || !it.accum || it.scene->ctx != c || it.accum->ctx != c) return fail(c, OXPT_E_ARG, "batch: item belongs to another context");
+        if (it.params.dims[0] != it.accum->w || it.params.dims[1] != it.accum->h)
+            return fail(c, OXPT_E_ARG, "batch: params size differs from the accumulator");
+        if (int rc = check_params(c, it.scene, &it.params)) return rc;
+        for (size_t k = 0; k < i; k++)
+            if (items[k].accum == it.accum) return fail(c, OXPT_E_ARG, "batch: two items share an accumulator");
+        const uint32_t *a = it.scene->host_tables.data(), *b = i0.scene->host_tables.data();
+        if (std::memcmp(a + it.params.offs0[0], b + i0.params.offs0[0], OXPT_SOBOL_WORDS * 4) != 0 ||
+            std::memcmp(a + it.params.offs0[1], b + i0.params.offs0[1], OXPT_SHEEN_WORDS * 4) != 0)
+            return fail(c, OXPT_E_ARG, "batch: jobs carry different Sobol' / sheen tables");
+    }
+    if (setdev(c)) return OXPT_E_BACKEND;
+    if (c->job_cap < n) {
+        (void)hipStreamSynchronize(c->stream);
+        if (c->graph) {
+            (void)hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+        }
+        dfree(c, c->d_jobs, c->job_cap * sizeof(oxd::Job));
+        dfree(c, c->d_counters, c->job_cap * 4 + 4);
+        c->d_jobs = nullptr;
+        c->d_counters = nullptr;
+        c->job_cap = 0;
+        size_t cap = std::max<size_t>(n, 256);
+        void *p = nullptr;
+        if (int rc = dalloc(c, cap * sizeof(oxd::Job), &p, "batch job table")) return rc;
+        c->d_jobs = (oxd::Job *)p;
+        if (int rc = dalloc(c, cap * 4 + 4, &p, "batch counters")) {
+            dfree(c, c->d_jobs, cap * sizeof(oxd::Job));
+            c->d_jobs = nullptr;
+            return rc;
+        }
+        c->d_counters = (uint32_t *)p;
+        c->job_cap = cap;
+    }
+    // Rounds: each round adds at most `spd` samples to every item that still
+    // has samples left, so one launch stays under max_paths paths.
+    uint64_t total = 0;
+    uint32_t max_count = 0;
+    for (size_t i = 0; i < n; i++) {
+        total += (uint64_t)items[i].accum->w * items[i].accum->h;
+        max_count = std::max(max_count, items[i].count);
+    }
+    uint32_t spd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_count, c->tuning.max_paths / std::max<uint64_t>(total, 1)));
+    uint32_t block = c->tuning.block;
+    uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
+    std::vector<oxd::Job> jobs;
+    std::vector<double> work;
+    for (uint32_t done = 0; done < max_count; done += spd) {
+        jobs.clear();
+        work.clear();
+        for (size_t i = 0; i < n; i++) {
+            const oxpt_batch_item &it = items[i];
+            if (it.count <= done) continue;
+            oxd::Job J;
+            std::memset(&J, 0, sizeof J);
+            J.P = it.params;
+            J.geom = it.scene->geom;
+            J.attrs = it.scene->attrs;
+            J.tables = it.scene->tables;
+            J.atlas = it.scene->atlas;
+            J.accum = it.accum->d;
+            J.atlas_side = it.scene->atlas_side;
+            J.first = it.first + done;
+            J.count = std::min(spd, it.count - done);
+            uint32_t tx = (it.accum->w + 7u) / 8u, ty = (it.accum->h + 7u) / 8u;
+            J.tiles_x = tx;
+            J.n_items = 64u * tx * ty;
+            jobs.push_back(J);
+            work.push_back((double)it.accum->w * it.accum->h * J.count);
+        }
+        // Largest job first: the small ones fill the tail.
+        std::vector<size_t> order(jobs.size());
+        for (size_t k = 0; k < order.size(); k++) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
+        std::vector<oxd::Job> sorted(jobs.size());
+        for (size_t k = 0; k < order.size(); k++) sorted[k] = jobs[order[k]];
+        uint32_t nj = (uint32_t)sorted.size();
+        hipError_t e = hipMemcpyAsync(c->d_jobs, sorted.data(), sorted.size() * sizeof(oxd::Job), hipMemcpyHostToDevice,
+                                      c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->d_counters + c->job_cap, &nj, 4, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return hip_fail(c, "batch job upload", e);
+        oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap};
+        EventPair ep{};
+        bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
+        if (timed) (void)hipEventRecord(ep.a, c->stream);
+        if (c->use_graph) {
+            if (c->graph && (c->graph_block != block || c->graph_grid != grid)) {
+                (void)hipGraphExecDestroy(c->graph);
+                c->graph = nullptr;
+            }
+            if (!c->graph) {
+                // Capture the step once: counter reset + trace, both reading
+                // the job count from device memory.
+                hipGraph_t g = nullptr;
+                e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    hipError_t le = oxd::launch_trace_batch(args, block, grid, c->stream);
+                    e = hipStreamEndCapture(c->stream, &g);
+                    if (e == hipSuccess) e = le;
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return hip_fail(c, "batch graph capture", e);
+                c->graph_block = block;
+                c->graph_grid = grid;
+            }
+            e = hipGraphLaunch(c->graph, c->stream);
+            c->graph_launches++;
+        } else {
+            e = oxd::launch_trace_batch(args, block, grid, c->stream);
+        }
+        if (e != hipSuccess) return hip_fail(c, "batch launch", e);
+        if (timed) {
+            (void)hipEventRecord(ep.b, c->stream);
+            c->events.push_back(ep);
+            c->launches++;
+        }
+        if (c->events.size() > 4096) fold_events(c, 2048);
+        // The host staging of the next round's table must not race this copy.
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return hip_fail(c, "batch", e);
     }
     return OXPT_OK;
 }

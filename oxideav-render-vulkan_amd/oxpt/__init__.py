@@ -130,6 +130,26 @@ class Tuning(C.Structure):
     _fields_ = [("block", u32), ("waves_per_cu", u32), ("max_paths", u32), ("kernel", u32)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("accum", C.c_void_p), ("params", Params), ("first", u32), ("count", u32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", u64), ("spp_per_tick", u32), ("max_active", u32), ("keep_hdr", u32),
+                ("use_graph", u32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("ok", u32), ("width", u32), ("height", u32), ("samples", u32), ("latency_s", C.c_double),
+                ("queue_s", C.c_double), ("paths", u64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("rejected", u64), ("ticks", u64), ("uploads", u64),
+                ("evictions", u64), ("evicted_accums", u64), ("paths", u64), ("busy_s", C.c_double),
+                ("resident_bytes", u64), ("active", u32), ("queued", u32)]
+
+
 ABI_STRUCTS = [Params, Frame, ResolveParams, SceneDesc, TexBinding, Material, Sampler, Light, Camera,
                COptions, Tuning]
 
@@ -163,6 +183,7 @@ _sig("oxpt_refine", C.c_int, _vp, _vp, _vp, P(Params), u32, u32)
 _sig("oxpt_finish", C.c_int, _vp)
 _sig("oxpt_resolve", C.c_int, _vp, _vp, P(ResolveParams), _vp, sz)
 _sig("oxpt_ctx_timing", C.c_int, _vp, C.c_int, P(u64), P(C.c_double))
+_sig("oxpt_refine_batch", C.c_int, _vp, P(BatchItem), sz)
 _sig("oxpt_ctx_set_tuning", C.c_int, _vp, P(Tuning))
 _sig("oxpt_ctx_get_tuning", C.c_int, _vp, P(Tuning))
 _sig("oxpt_debug_pattern_words", C.c_int, _vp, _vp, u32, sz, P(u32), P(u32), P(u32), P(u32), P(u32))
@@ -215,6 +236,20 @@ _sig("oxpt_packed_words", C.c_int, _vp, C.c_int, P(_vp), P(sz))
 _sig("oxpt_packed_params", C.c_int, _vp, P(Params))
 _sig("oxpt_packed_info", C.c_int, _vp, P(u32), P(u32), P(u32))
 _sig("oxpt_registry_make", C.c_int, C.c_char_p, P(_vp))
+_sig("oxpt_ctx_set_graph", C.c_int, _vp, C.c_int)
+_sig("oxpt_ctx_graph_launches", u64, _vp)
+_sig("oxpt_server_config_default", None, P(ServerConfig))
+_sig("oxpt_server_create", C.c_int, C.c_int, P(ServerConfig), P(_vp))
+_sig("oxpt_server_destroy", None, _vp)
+_sig("oxpt_server_ctx", _vp, _vp)
+_sig("oxpt_server_last_error", C.c_char_p, _vp)
+_sig("oxpt_server_add_scene", C.c_int, _vp, _vp, _vp, P(u32))
+_sig("oxpt_server_submit_render", C.c_int, _vp, u32, P(COptions), P(u64))
+_sig("oxpt_server_submit_refine", C.c_int, _vp, u64, u32, P(u64))
+_sig("oxpt_server_wait", C.c_int, _vp, u64, C.c_double, P(RequestInfo))
+_sig("oxpt_server_result", C.c_int, _vp, u64, C.c_int, _vp, sz)
+_sig("oxpt_server_release", C.c_int, _vp, u64)
+_sig("oxpt_server_stats_get", C.c_int, _vp, P(ServerStats))
 _sig("oxpt_abi_sizes", None, P(u32))
 _sig("oxpt_f32_to_f16", C.c_uint16, f32)
 _sig("oxpt_sobol_directions", None, P(u32))
@@ -577,6 +612,185 @@ class GpuPathTracer:
         if rc:
             _raise(rc, lib.oxpt_last_error(self.ctx).decode())
         return out
+
+
+class Device:
+    """The device boundary (layer 1): resident scenes and accumulators on one context."""
+
+    def __init__(self, ordinal: int = 0):
+        h = _vp()
+        rc = lib.oxpt_ctx_create(ordinal, C.byref(h))
+        if rc:
+            _raise(rc, "no HIP device")
+        self.h = h.value
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.oxpt_ctx_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc, what):
+        if rc:
+            _raise(rc, f"{what}: {lib.oxpt_last_error(self.h).decode()}")
+
+    def summary(self) -> str:
+        return lib.oxpt_ctx_summary(self.h).decode()
+
+    def upload(self, pk: "Packed") -> "DeviceScene":
+        return DeviceScene(self, pk)
+
+    def accum(self, w: int, h: int) -> "Accum":
+        return Accum(self, w, h)
+
+    def refine(self, scene: "DeviceScene", acc: "Accum", params: Params, first: int, count: int) -> None:
+        self._check(lib.oxpt_refine(self.h, scene.h, acc.h, C.byref(params), first, count), "refine")
+
+    def refine_batch(self, items) -> None:
+        """items: iterable of (DeviceScene, Accum, Params, first, count)."""
+        arr = (BatchItem * len(items))()
+        for i, (sc, acc, prm, first, count) in enumerate(items):
+            arr[i] = BatchItem(sc.h, acc.h, prm, first, count)
+        self._check(lib.oxpt_refine_batch(self.h, arr, len(items)), "refine_batch")
+
+    def finish(self) -> None:
+        self._check(lib.oxpt_finish(self.h), "finish")
+
+    def timing(self, reset: bool = False):
+        n, ms = u64(), C.c_double()
+        lib.oxpt_ctx_timing(self.h, 1 if reset else 0, C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
+    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel)
+        self._check(lib.oxpt_ctx_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        lib.oxpt_ctx_get_tuning(self.h, C.byref(t))
+        return t
+
+    def set_budget(self, nbytes: int) -> None:
+        self._check(lib.oxpt_ctx_set_budget(self.h, nbytes), "set_budget")
+
+    def bytes_in_use(self) -> int:
+        return lib.oxpt_ctx_bytes_in_use(self.h)
+
+
+class DeviceScene:
+    def __init__(self, dev: Device, pk: "Packed"):
+        self.dev = dev
+        d = SceneDesc()
+        d.geom, d.geom_vec4 = _ptr(pk.geom, f32), pk.geom.size // 4
+        d.attrs, d.attrs_vec4 = _ptr(pk.attrs, f32), pk.attrs.size // 4
+        d.tables, d.table_words = _ptr(pk.tables, u32), pk.tables.size
+        d.atlas = _ptr(pk.atlas, C.c_uint16) if pk.atlas.size else None
+        d.atlas_side, d.atlas_layers = pk.atlas_side, pk.atlas_layers
+        h = _vp()
+        dev._check(lib.oxpt_scene_upload(dev.h, C.byref(d), C.byref(h)), "scene_upload")
+        self.h = h.value
+        self.params = pk.params
+
+    def __del__(self):
+        if getattr(self, "h", None) and getattr(self.dev, "h", None):
+            lib.oxpt_scene_destroy(self.h)
+            self.h = None
+
+
+class Accum:
+    def __init__(self, dev: Device, w: int, h: int):
+        self.dev, self.w, self.h_ = dev, w, h
+        hh = _vp()
+        dev._check(lib.oxpt_accum_create(dev.h, w, h, C.byref(hh)), "accum_create")
+        self.h = hh.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and getattr(self.dev, "h", None):
+            lib.oxpt_accum_destroy(self.h)
+            self.h = None
+
+    def clear(self) -> None:
+        self.dev._check(lib.oxpt_accum_clear(self.dev.h, self.h), "accum_clear")
+
+    def read(self) -> np.ndarray:
+        out = np.zeros((self.h_, self.w, 4), dtype=np.float32)
+        self.dev._check(lib.oxpt_accum_read(self.dev.h, self.h, _ptr(out, f32), out.nbytes), "accum_read")
+        return out
+
+    def resolve(self, rp: ResolveParams) -> np.ndarray:
+        hdr = rp.cfg[3] == 1
+        out = np.zeros((self.h_, self.w, 4), dtype=np.float32 if hdr else np.uint8)
+        self.dev._check(lib.oxpt_resolve(self.dev.h, self.h, C.byref(rp), out.ctypes.data_as(C.c_void_p), out.nbytes),
+                        "resolve")
+        return out
+
+
+class Server:
+    """Continuous-batching render server over one device (include/oxpt/oxpt.h, serving)."""
+
+    def __init__(self, device=0, hbm_budget=200 << 30, spp_per_tick=16, max_active=0, keep_hdr=False,
+                 use_graph=False):
+        cfg = ServerConfig()
+        lib.oxpt_server_config_default(C.byref(cfg))
+        cfg.hbm_budget, cfg.spp_per_tick, cfg.max_active = hbm_budget, spp_per_tick, max_active
+        cfg.keep_hdr, cfg.use_graph = 1 if keep_hdr else 0, 1 if use_graph else 0
+        h = _vp()
+        rc = lib.oxpt_server_create(device, C.byref(cfg), C.byref(h))
+        if rc:
+            _raise(rc, "server: no HIP device")
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.oxpt_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            _raise(rc, f"{what}: {lib.oxpt_server_last_error(self.h).decode()}")
+
+    def add_scene(self, scene: Scene, env: Optional[EnvironmentMap] = None) -> int:
+        i = u32()
+        self._check(lib.oxpt_server_add_scene(self.h, scene.h, env.h if env else None, C.byref(i)), "add_scene")
+        return i.value
+
+    def submit_render(self, scene_id: int, opts: RenderOptions) -> int:
+        r = u64()
+        o = opts.to_c()
+        self._check(lib.oxpt_server_submit_render(self.h, scene_id, C.byref(o), C.byref(r)), "submit_render")
+        return r.value
+
+    def submit_refine(self, render_req: int, spp: int) -> int:
+        r = u64()
+        self._check(lib.oxpt_server_submit_refine(self.h, render_req, spp, C.byref(r)), "submit_refine")
+        return r.value
+
+    def wait(self, req: int, timeout: float = -1.0) -> RequestInfo:
+        info = RequestInfo()
+        self._check(lib.oxpt_server_wait(self.h, req, timeout, C.byref(info)), f"wait({req})")
+        return info
+
+    def result(self, req: int, hdr: bool = False) -> np.ndarray:
+        info = RequestInfo()
+        lib.oxpt_server_wait(self.h, req, 0.0, C.byref(info))
+        out = np.zeros((info.height, info.width, 4), dtype=np.float32 if hdr else np.uint8)
+        self._check(lib.oxpt_server_result(self.h, req, 1 if hdr else 0, out.ctypes.data_as(C.c_void_p), out.nbytes),
+                    "result")
+        return out
+
+    def release(self, req: int) -> None:
+        lib.oxpt_server_release(self.h, req)
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        lib.oxpt_server_stats_get(self.h, C.byref(st))
+        return st
+
+    def graph_launches(self) -> int:
+        return lib.oxpt_ctx_graph_launches(lib.oxpt_server_ctx(self.h))
 
 
 class RenderRegistry:
