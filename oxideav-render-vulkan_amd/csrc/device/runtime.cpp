@@ -181,7 +181,13 @@ int oxpt_ctx_create(int ordinal, oxpt_ctx **out) {
     }
     c->n_cu = (uint32_t)prop.multiProcessorCount;
     c->budget = (uint64_t)prop.totalGlobalMem;
-    c->summary = std::string(prop.name) + " (" + prop.gcnArchName + ", HIP)";
+    std::string name = prop.name;
+    if (name.empty()) {
+        char buf[256] = {0};
+        if (hipDeviceGetName(buf, sizeof buf, ordinal) == hipSuccess) name = buf;
+    }
+    if (name.empty()) name = "AMD GPU";
+    c->summary = name + " (" + prop.gcnArchName + ", " + std::to_string(c->n_cu) + " CUs, HIP)";
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return OXPT_E_BACKEND;

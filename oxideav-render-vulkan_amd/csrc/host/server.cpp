@@ -81,7 +81,7 @@ struct oxpt_server {
     std::deque<uint64_t> queue;   // admission order
     std::vector<uint64_t> active;
     uint64_t next_id = 1, tick_no = 0, use_clock = 0;
-    uint64_t submit_gen = 0;  // bumped by every submission
+    uint64_t change_gen = 0;  // bumped by every submission and completion
     uint64_t resident = 0;
     oxpt_server_stats stats{};
 
@@ -216,11 +216,12 @@ void oxpt_server::run() {
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(mu);
-            // Sleep until there is something to run, or a new submission to
-            // try admitting (a blocked queue with nothing active waits here).
-            cv_work.wait(lk, [&] { return stop || !active.empty() || (!queue.empty() && submit_gen != seen_gen); });
+            // Sleep until there is something to run, or something changed that
+            // may let a queued request in (a submission, or a completion that
+            // freed its scene or finished the render a refine waits for).
+            cv_work.wait(lk, [&] { return stop || !active.empty() || (!queue.empty() && change_gen != seen_gen); });
             if (stop) return;
-            seen_gen = submit_gen;
+            seen_gen = change_gen;
             // Admit in arrival order while slots and memory allow.
             while (!queue.empty() && (cfg.max_active == 0 || active.size() < cfg.max_active)) {
                 Req &r = *reqs[queue.front()];
@@ -290,6 +291,7 @@ void oxpt_server::run() {
             else still.push_back(r.id);
         }
         active.swap(still);
+        change_gen++;
         cv_done.notify_all();
     }
 }
@@ -379,7 +381,7 @@ int oxpt_server_submit_render(oxpt_server *s, uint32_t scene_id, const oxpt_rend
     s->queue.push_back(r->id);
     s->reqs[r->id] = std::move(r);
     s->stats.submitted++;
-    s->submit_gen++;
+    s->change_gen++;
     s->cv_work.notify_all();
     return OXPT_OK;
 }
@@ -403,7 +405,7 @@ int oxpt_server_submit_refine(oxpt_server *s, uint64_t render_req, uint32_t spp,
     s->queue.push_back(r->id);
     s->reqs[r->id] = std::move(r);
     s->stats.submitted++;
-    s->submit_gen++;
+    s->change_gen++;
     s->cv_work.notify_all();
     return OXPT_OK;
 }

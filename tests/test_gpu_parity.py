@@ -22,6 +22,7 @@ def small_terrain():
 def test_pattern_words_bit_exact(gpu):
     scene = oxpt.Scene.room(0)
     opts = oxpt.RenderOptions(width=64, height=64, samples_per_pixel=1)
+    gpu.invalidate_scene()
     gpu.sync(scene, opts)
     gpu.refine(1)
     rng = np.random.default_rng(7)
@@ -43,6 +44,7 @@ def test_light_cdf_indices_bit_exact(gpu):
     env = oxpt.EnvironmentMap.sky(256, 128, 5)
     opts = oxpt.RenderOptions(width=32, height=32, samples_per_pixel=1, max_bounces=4)
     gpu.set_environment(env)
+    gpu.invalidate_scene()  # a different Scene3D must be signalled (README: sync rules)
     gpu.sync(scene, opts)
     gpu.refine(1)
     pk = oxpt.Packed(scene, opts, env)
@@ -69,6 +71,7 @@ def test_closest_hit_matches_oracle(gpu):
     scene = small_terrain()
     opts = oxpt.RenderOptions(width=32, height=32, samples_per_pixel=1)
     gpu.set_environment(None)
+    gpu.invalidate_scene()
     gpu.sync(scene, opts)
     gpu.refine(1)
     pk = oxpt.Packed(scene, opts)
@@ -138,3 +141,38 @@ def test_plain_kernel_matches_persistent(gpu):
     gpu.set_tuning(kernel=0)
     b = gpu.render_hdr(scene, opts)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# The reference's parity scenes (tests/pathtrace.rs:319-418): every lobe,
+# punctual lights, alpha modes, textures with mips, direct-only.
+# ---------------------------------------------------------------------------
+
+import ptscenes  # noqa: E402
+
+
+@pytest.mark.parametrize("name,bounces,spp", [("materials", 8, 32), ("alpha", 8, 32), ("checker", 4, 16),
+                                              ("spheres", 8, 16), ("materials-direct", 1, 32)])
+def test_reference_scene_parity(gpu, name, bounces, spp):
+    scene = {"materials": ptscenes.materials_scene, "materials-direct": ptscenes.materials_scene,
+             "alpha": ptscenes.alpha_planes, "checker": ptscenes.checker_floor,
+             "spheres": ptscenes.sphere_grid}[name]()
+    opts = oxpt.RenderOptions(width=128, height=96, samples_per_pixel=spp, max_bounces=bounces)
+    gpu.set_environment(None)
+    gh = gpu.render_hdr(scene, opts)
+    gi = gpu.image()
+    _, ci, ch = oracle_frame(scene, opts)
+    p = parity(name, gh, gi, ch, ci, 60.0)
+    print(f"{name}: PSNR {p:.2f} dB")
+
+
+def test_layered_texture_scene_parity(gpu):
+    """Layered BSDF + textures + 16 punctual lights (the large layered workload, scaled down)."""
+    scene = oxpt.Scene.layered(seed=2, n_tris=40000, n_textures=4, tex_size=256, n_lights=16)
+    opts = oxpt.RenderOptions(width=128, height=72, samples_per_pixel=8, max_bounces=16)
+    gpu.set_environment(None)
+    gh = gpu.render_hdr(scene, opts)
+    gi = gpu.image()
+    _, ci, ch = oracle_frame(scene, opts)
+    p = parity("layered", gh, gi, ch, ci, 60.0)
+    print(f"layered: PSNR {p:.2f} dB")

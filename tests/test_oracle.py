@@ -371,3 +371,25 @@ def test_resolve_background_and_tone_maps():
     rp = oxpt.ResolveParams((1, 3, 0, 1), (1.0, 4.0, 3.0, 0.0), tuple(c / 255.0 for c in bg))
     hdr = oracle.resolve(acc, rp)
     assert np.allclose(hdr[0, 1, :3], np.array([0.5, 0.25, 2.0]) / 4.0)
+
+
+# ---------------------------------------------------------------------------
+# The reference's parity scenes render finitely on the oracle (sanity of the
+# scene builders the GPU parity tests use).
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["materials", "alpha", "checker", "spheres"])
+def test_reference_scenes_render_on_the_oracle(name):
+    import ptscenes
+
+    scene = {"materials": ptscenes.materials_scene, "alpha": ptscenes.alpha_planes,
+             "checker": ptscenes.checker_floor, "spheres": ptscenes.sphere_grid}[name]()
+    opts = oxpt.RenderOptions(width=48, height=32, samples_per_pixel=4, max_bounces=6)
+    osc = oracle.Scene.from_packed(oxpt.Packed(scene, opts))
+    acc, cnt = osc.render(0, 4, counts=True)
+    assert np.isfinite(acc).all()
+    assert (acc[..., 3] > 0).mean() > 0.5
+    m = acc[..., :3].sum() / max(acc[..., 3].sum(), 1)
+    assert 0.005 < m < 50, m
+    if name == "checker":
+        assert cnt.tex_taps > 0
