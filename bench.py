@@ -68,18 +68,96 @@ def tiles(w, h, n, size=64, seed=1234):
     return out
 
 
+def bench_uneven_batch(args):
+    """64 scenes of uneven size: per-request plain launches vs one batched launch."""
+    import oxpt
+    from oxpt import workloads
+
+    t = time.perf_counter()
+    jobs = workloads.uneven_batch(n=64, seed=3, threads=min(16, os.cpu_count() or 1))
+    log(f"64 scenes generated + packed in {time.perf_counter() - t:.1f} s "
+        f"({sum(j.n_tris for j in jobs)} tris, {sum(j.paths for j in jobs) / 1e6:.1f} M paths)")
+    dev = oxpt.Device(0)
+    log("device:", dev.summary())
+    ds = [dev.upload(j.packed) for j in jobs]
+    acc = [dev.accum(j.opts.width, j.opts.height) for j in jobs]
+    total_paths = sum(j.paths for j in jobs)
+    items = [(s, a, s.params, 0, j.opts.samples_per_pixel) for j, s, a in zip(jobs, ds, acc)]
+
+    def plain():
+        dev.set_tuning(kernel=1)
+        for j, s, a in zip(jobs, ds, acc):
+            dev.refine(s, a, s.params, 0, j.opts.samples_per_pixel)
+        dev.finish()
+
+    def batched():
+        dev.set_tuning(kernel=args.kernel if args.kernel != 1 else 0, block=args.block,
+                       waves_per_cu=args.waves_per_cu)
+        dev.refine_batch(items)
+        dev.finish()
+
+    res = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        for _ in range(args.warmup):
+            for a in acc:
+                a.clear()
+            fn()
+        times = []
+        for i in range(args.steps):
+            for a in acc:
+                a.clear()
+            dev.finish()
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+            log(f"{name} {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
+        res[name] = (sorted(times)[len(times) // 2], [a.read() for a in acc] if name == "plain" else None)
+    same = all(np_equal(a.read(), b) for a, b in zip(acc, res["plain"][1]))
+    tp, tb = res["plain"][0], res["batched"][0]
+    out = {
+        "metric": "Mpaths/s on the uneven batch (64 scenes): batched persistent launch vs per-request plain kernel",
+        "value": round(total_paths / tb / 1e6, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(tb * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene generators)",
+        "config": {"workload": "uneven batch: 64 terrain scenes, tris log-uniform 1k..2M, side U(128..1024), "
+                               "spp in {4,16,64,256}, depth 12, per-scene seeds", "paths": total_paths},
+        "plain_mpaths_s": round(total_paths / tp / 1e6, 3), "speedup_vs_plain": round(tp / tb, 3),
+        "bitwise_equal_to_plain": bool(same),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def np_equal(a, b):
+    return a.shape == b.shape and bool((a.view(np.uint32) == b.view(np.uint32)).all())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3"])
     ap.add_argument("--cpu-tiles", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg (PSNR, baseline, bytes/path)")
     ap.add_argument("--kernel", type=int, default=0, help="0 persistent megakernel, 1 plain per-pixel kernel")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--wf-variant", type=int, default=0, help="wavefront kernel forms (kernel 7)")
+    ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (profiling runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device with a plain copy kernel (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_calibration} MiB ({n * 4} bytes read, {n * 4} bytes written)")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the path shards by sample range or scene, see DESIGN.md)", file=sys.stderr)
@@ -87,10 +165,14 @@ def main():
 
     import oxpt
 
+    if args.workload == "c3":
+        return bench_uneven_batch(args)
     scene, env, opts, desc = workload(args.workload)
     pt = oxpt.GpuPathTracer(0)
     log("device:", pt.adapter_summary())
-    pt.set_tuning(block=args.block, waves_per_cu=args.waves_per_cu, kernel=args.kernel)
+    pt.set_tuning(block=args.block, waves_per_cu=args.waves_per_cu, kernel=args.kernel, wf_variant=args.wf_variant)
+    if args.spp:
+        opts.samples_per_pixel = args.spp
     t = time.perf_counter()
     if env is not None:
         pt.set_environment(env)

@@ -373,7 +373,11 @@ __device__ __forceinline__ void item_pixel(uint32_t tiles_x, uint32_t item, uint
     y = ty * 8u + (in >> 3);
 }
 
-__global__ void __launch_bounds__(256) trace_persistent(TraceArgs A) {
+// MinWaves > 0 asks the compiler for that many waves per SIMD (fewer
+// VGPRs, more latency hiding, possible spills): an autotuner knob.
+template <int MinWaves>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves > 0 ? MinWaves : 1)))
+trace_persistent(TraceArgs A) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
     stage_tables(A.tables, A.P, lds_sobol, lds_sheen);
@@ -471,7 +475,9 @@ __global__ void __launch_bounds__(64) trace_tiles(TraceArgs A) {
 // next job that still has work. Jobs are ordered largest first by the
 // host, so the small ones fill the tail. Per-pixel sample order is kept,
 // so every job's image equals its image alone.
-__global__ void __launch_bounds__(256) trace_batch(BatchArgs B) {
+template <int MinWaves>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves)))
+trace_batch(BatchArgs B) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
     // Every job carries the same estimator constants (checked on the host).
@@ -680,15 +686,29 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s, args.work);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(trace_persistent, dim3(grid), dim3(block), 0, s, args);
+    switch (kernel) {
+        case 2: hipLaunchKernelGGL(trace_persistent<3>, dim3(grid), dim3(block), 0, s, args); break;
+        case 3: hipLaunchKernelGGL(trace_persistent<4>, dim3(grid), dim3(block), 0, s, args); break;
+        case 4: hipLaunchKernelGGL(trace_persistent<5>, dim3(grid), dim3(block), 0, s, args); break;
+        case 5: hipLaunchKernelGGL(trace_persistent<6>, dim3(grid), dim3(block), 0, s, args); break;
+        case 6: hipLaunchKernelGGL(trace_persistent<8>, dim3(grid), dim3(block), 0, s, args); break;
+        default: hipLaunchKernelGGL(trace_persistent<0>, dim3(grid), dim3(block), 0, s, args); break;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_trace_batch(const BatchArgs &args, uint32_t block, uint32_t grid, hipStream_t s) {
+hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, hipStream_t s) {
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, s, args.counters, args.n_jobs, args.max_jobs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(trace_batch, dim3(grid), dim3(block), 0, s, args);
+    switch (kernel) {
+        case 2: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), 0, s, args); break;
+        case 3: hipLaunchKernelGGL(trace_batch<4>, dim3(grid), dim3(block), 0, s, args); break;
+        case 4: hipLaunchKernelGGL(trace_batch<5>, dim3(grid), dim3(block), 0, s, args); break;
+        case 5: hipLaunchKernelGGL(trace_batch<6>, dim3(grid), dim3(block), 0, s, args); break;
+        case 6: hipLaunchKernelGGL(trace_batch<8>, dim3(grid), dim3(block), 0, s, args); break;
+        default: hipLaunchKernelGGL(trace_batch<2>, dim3(grid), dim3(block), 0, s, args); break;
+    }
     return hipGetLastError();
 }
 

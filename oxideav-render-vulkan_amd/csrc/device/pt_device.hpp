@@ -384,10 +384,32 @@ __device__ __forceinline__ bool accept(const Scene &S, uint32_t slot, F3 b, bool
     return coin(path_seed, ray_id, gid) < hit_alpha(S, slot, b);
 }
 
-// Ordered while-while walk (wgsl:651-735). The per-lane stack lives in
-// private memory, which the hardware interleaves per lane.
+// Traversal stack: 64 entries (the reference's depth, wgsl:661-662), each
+// (t_far bits << 32 | node). With LdsDepth > 0 the top LdsDepth entries of
+// every lane live in LDS — laid out [entry][lane] in the wave's own slice so
+// lanes at the same depth touch consecutive words — and deeper entries go to
+// private memory; with LdsDepth == 0 the whole stack is private.
+template <int LdsDepth>
+struct TravStack {
+    uint64_t *lds;  // this wave's slice: LdsDepth * 64 entries (unused when 0)
+    uint32_t lane;
+    uint64_t priv[STACK_DEPTH - LdsDepth];
+    __device__ __forceinline__ void put(uint32_t i, uint64_t v) {
+        if (LdsDepth > 0 && i < (uint32_t)LdsDepth) lds[i * 64u + lane] = v;
+        else priv[i - LdsDepth] = v;
+    }
+    __device__ __forceinline__ uint64_t get(uint32_t i) const {
+        if (LdsDepth > 0 && i < (uint32_t)LdsDepth) return lds[i * 64u + lane];
+        return priv[i - LdsDepth];
+    }
+};
+
+// Ordered while-while walk (wgsl:651-735): near child first, far child
+// pushed with its entry distance, pops culled by t_max.
+template <int LdsDepth = 0>
 __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
-                                        uint32_t ray_id, bool primary, bool any, uint32_t skip) {
+                                        uint32_t ray_id, bool primary, bool any, uint32_t skip,
+                                        uint64_t *lds_stack = nullptr) {
     Hit best;
     best.slot = NONE;
     best.t = 0.0f;
@@ -396,7 +418,9 @@ __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_ma
     if (S.node_count == 0 || !r.valid) return best;
     float t_max = t_max_in;
     if (slab(r, S.geom[0], S.geom[1], t_max) < 0.0f) return best;
-    uint64_t stack[STACK_DEPTH];  // (t bits << 32) | node
+    TravStack<LdsDepth> stack;
+    stack.lds = lds_stack;
+    stack.lane = __lane_id();
     uint32_t sp = 0, idx = 0;
     for (;;) {
         float4 lo = S.geom[2u * idx], hi = S.geom[2u * idx + 1u];
@@ -441,7 +465,7 @@ __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_ma
                     t_far = tl;
                 }
                 if (sp < STACK_DEPTH) {
-                    stack[sp] = ((uint64_t)fbits(t_far) << 32) | far_;
+                    stack.put(sp, ((uint64_t)fbits(t_far) << 32) | far_);
                     sp++;
                 }
                 idx = near_;
@@ -458,7 +482,7 @@ __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_ma
         bool found = false;
         while (sp != 0) {
             sp--;
-            uint64_t e = stack[sp];
+            uint64_t e = stack.get(sp);
             if (bitsf((uint32_t)(e >> 32)) <= t_max) {
                 idx = (uint32_t)e;
                 found = true;

@@ -13,6 +13,7 @@
 
 #include "oxpt/oxpt.h"
 #include "pt_kernels.hpp"
+#include "wavefront.hpp"
 
 namespace {
 
@@ -32,20 +33,37 @@ struct oxpt_ctx {
     std::string summary;
     uint64_t budget = 0, in_use = 0;
     uint32_t n_cu = 0;
-    oxpt_tuning tuning{256, 32, 1u << 27, 0};
+    oxpt_tuning tuning{256, 32, 1u << 27, 0, 0, {0, 0, 0}};
     uint32_t *d_work = nullptr;
     oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[job_cap]
     uint32_t *d_counters = nullptr;  // job_cap counters + 1 count word
     size_t job_cap = 0;
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;  // captured batched step (reset + trace)
-    uint32_t graph_block = 0, graph_grid = 0;
+    uint32_t graph_block = 0, graph_grid = 0, graph_kernel = 0;
     uint64_t graph_launches = 0;
     void *d_out = nullptr;
     size_t d_out_bytes = 0;
     std::vector<EventPair> events;
     uint64_t launches = 0;
     double total_ms = 0.0;
+    // Wavefront buffers (grown on demand).
+    struct Wf {
+        oxd::PathSlot *slots = nullptr;
+        oxd::HitRec *hits = nullptr;
+        oxd::ShadowRec *shadows = nullptr;
+        uint32_t *occluded = nullptr;
+        float4 *samples = nullptr;
+        oxd::Job *jobs = nullptr;
+        uint32_t *job_base = nullptr;
+        uint32_t *counters = nullptr;   // [0] item counter, [1..] live ring
+        uint32_t *host_live = nullptr;  // pinned copy of the live ring
+        std::vector<hipEvent_t> events;
+        size_t slot_bytes = 0, hit_bytes = 0, shadow_bytes = 0, occ_bytes = 0, sample_bytes = 0, job_bytes = 0,
+               base_bytes = 0;
+        uint32_t n_slots = 0, max_shadow = 0;
+        uint64_t iterations = 0;
+    } wf;
 };
 
 struct oxpt_scene {
@@ -158,6 +176,124 @@ int check_params(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P) {
 
 }  // namespace
 
+namespace {
+
+// (Re)allocate a device buffer to hold at least `need` bytes.
+template <typename T>
+int wf_grow(oxpt_ctx *c, T *&p, size_t &have, size_t need, const char *what) {
+    if (have >= need && p) return OXPT_OK;
+    dfree(c, p, have);
+    p = nullptr;
+    have = 0;
+    void *q = nullptr;
+    if (int rc = dalloc(c, need, &q, what)) return rc;
+    p = static_cast<T *>(q);
+    have = need;
+    return OXPT_OK;
+}
+
+// Wavefront engine: traces every job's [first, first + count) samples in
+// passes of at most 2^27 (pixel, sample) items (max_paths caps it further),
+// then adds each pass's samples into the accumulators in sample order.
+int wf_run(oxpt_ctx *c, const std::vector<oxd::Job> &jobs) {
+    auto &W = c->wf;
+    uint32_t max_shadow = 0, max_count = 0;
+    uint64_t pixels = 0;
+    for (auto &J : jobs) {
+        max_shadow = std::max(max_shadow, J.P.cfg[2] + 1u);
+        pixels += (uint64_t)J.n_items;  // 64 * tiles (incl. ragged edges)
+        max_count = std::max(max_count, J.count);
+    }
+    const uint64_t cap = std::min<uint64_t>(c->tuning.max_paths, 1ull << 27);
+    uint32_t spd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_count, cap / std::max<uint64_t>(pixels, 1)));
+    uint64_t items_max = 0;
+    for (auto &J : jobs) items_max += (uint64_t)J.n_items * std::min(spd, J.count);
+    if (items_max == 0) return OXPT_OK;
+    if (items_max >= 0xffffffffull) return fail(c, OXPT_E_ARG, "wavefront: pass too large");
+    uint32_t want_slots = (uint32_t)std::min<uint64_t>(1ull << 21, (items_max + 255) / 256 * 256);
+    if (W.n_slots < want_slots || W.max_shadow < max_shadow) {
+        uint32_t ns = std::max(want_slots, W.n_slots), ms = std::max(max_shadow, W.max_shadow);
+        if (int rc = wf_grow(c, W.slots, W.slot_bytes, (size_t)ns * sizeof(oxd::PathSlot), "wavefront slots")) return rc;
+        if (int rc = wf_grow(c, W.hits, W.hit_bytes, (size_t)ns * sizeof(oxd::HitRec), "wavefront hits")) return rc;
+        if (int rc = wf_grow(c, W.shadows, W.shadow_bytes, (size_t)ns * ms * sizeof(oxd::ShadowRec), "wavefront shadow rays"))
+            return rc;
+        if (int rc = wf_grow(c, W.occluded, W.occ_bytes, (size_t)ns * ms * 4, "wavefront occlusion")) return rc;
+        W.n_slots = ns;
+        W.max_shadow = ms;
+    }
+    if (int rc = wf_grow(c, W.samples, W.sample_bytes, items_max * 16, "wavefront samples")) return rc;
+    if (int rc = wf_grow(c, W.jobs, W.job_bytes, jobs.size() * sizeof(oxd::Job), "wavefront jobs")) return rc;
+    if (int rc = wf_grow(c, W.job_base, W.base_bytes, jobs.size() * 4, "wavefront job bases")) return rc;
+    if (!W.counters) {
+        void *q = nullptr;
+        if (int rc = dalloc(c, 4 * (1 + oxd::WF_LIVE_RING), &q, "wavefront counters")) return rc;
+        W.counters = (uint32_t *)q;
+    }
+    if (!W.host_live && hipHostMalloc((void **)&W.host_live, 4 * oxd::WF_LIVE_RING, hipHostMallocDefault) != hipSuccess) {
+        W.host_live = nullptr;
+        return fail(c, OXPT_E_BACKEND, "hip: pinned wavefront counters");
+    }
+    if (W.events.empty()) {
+        W.events.assign(oxd::WF_LIVE_RING, nullptr);
+        for (auto &e : W.events)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(c, OXPT_E_BACKEND, "hip: events");
+    }
+    for (uint32_t done = 0; done < max_count; done += spd) {
+        std::vector<oxd::Job> pass;
+        std::vector<uint32_t> base;
+        uint64_t n_items = 0;
+        for (auto &J0 : jobs) {
+            if (J0.count <= done) continue;
+            oxd::Job J = J0;
+            J.first = J0.first + done;
+            J.count = std::min(spd, J0.count - done);
+            base.push_back((uint32_t)n_items);
+            n_items += (uint64_t)J.n_items * J.count;
+            pass.push_back(J);
+        }
+        hipError_t e = hipMemcpyAsync(W.jobs, pass.data(), pass.size() * sizeof(oxd::Job), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(W.job_base, base.data(), base.size() * 4, hipMemcpyHostToDevice, c->stream);
+        oxd::WfArgs A{};
+        A.jobs = W.jobs;
+        A.job_base = W.job_base;
+        A.n_jobs = (uint32_t)pass.size();
+        A.slots = W.slots;
+        A.hits = W.hits;
+        A.shadows = W.shadows;
+        A.occluded = W.occluded;
+        A.samples = W.samples;
+        A.item_counter = W.counters;
+        A.live = W.counters + 1;
+        A.n_slots = (uint32_t)std::min<uint64_t>(W.n_slots, (n_items + 255) / 256 * 256);
+        A.max_shadow = max_shadow;
+        A.n_items = (uint32_t)n_items;
+        if (e == hipSuccess) e = oxd::wf_launch_clear(A, c->stream);
+        // Iterate until a shade pass finds no live slot. The check lags by a
+        // few iterations so the host never waits on the newest launch.
+        const uint32_t LAG = 6;
+        for (uint32_t it = 0; e == hipSuccess; it++) {
+            A.iter = it;
+            e = oxd::wf_launch_iteration(A, c->tuning.wf_variant, c->stream);
+            if (e != hipSuccess) break;
+            uint32_t r = it & (oxd::WF_LIVE_RING - 1);
+            e = hipMemcpyAsync(W.host_live + r, A.live + r, 4, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(W.events[r], c->stream);
+            W.iterations++;
+            if (e == hipSuccess && it >= LAG) {
+                uint32_t q = (it - LAG) & (oxd::WF_LIVE_RING - 1);
+                if ((e = hipEventSynchronize(W.events[q])) != hipSuccess) break;
+                if (W.host_live[q] == 0) break;
+            }
+        }
+        for (uint32_t k = 0; e == hipSuccess && k < pass.size(); k++) e = oxd::wf_launch_reduce(A, k, pass[k].n_items, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return hip_fail(c, "wavefront", e);
+    }
+    return OXPT_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int oxpt_ctx_create(int ordinal, oxpt_ctx **out) {
@@ -210,6 +346,20 @@ void oxpt_ctx_destroy(oxpt_ctx *c) {
     fold_events(c, 0);
     dfree(c, c->d_work, 256);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    {
+        auto &W = c->wf;
+        dfree(c, W.slots, W.slot_bytes);
+        dfree(c, W.hits, W.hit_bytes);
+        dfree(c, W.shadows, W.shadow_bytes);
+        dfree(c, W.occluded, W.occ_bytes);
+        dfree(c, W.samples, W.sample_bytes);
+        dfree(c, W.jobs, W.job_bytes);
+        dfree(c, W.job_base, W.base_bytes);
+        dfree(c, W.counters, 4 * (1 + oxd::WF_LIVE_RING));
+        if (W.host_live) (void)hipHostFree(W.host_live);
+        for (auto &e : W.events)
+            if (e) (void)hipEventDestroy(e);
+    }
     dfree(c, c->d_jobs, c->job_cap * sizeof(oxd::Job));
     dfree(c, c->d_counters, c->job_cap * 4 + 4);
     dfree(c, c->d_out, c->d_out_bytes);
@@ -236,6 +386,7 @@ int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
     if (t->waves_per_cu) c->tuning.waves_per_cu = std::min(t->waves_per_cu, 64u);
     if (t->max_paths) c->tuning.max_paths = t->max_paths;
     c->tuning.kernel = t->kernel;
+    c->tuning.wf_variant = t->wf_variant;
     return OXPT_OK;
 }
 int oxpt_ctx_set_graph(oxpt_ctx *c, int enable) {
@@ -407,6 +558,31 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     if (count == 0) return OXPT_OK;
     if (int rc = check_params(c, s, P)) return rc;
     if (setdev(c)) return OXPT_E_BACKEND;
+    if (c->tuning.kernel == 7) {
+        oxd::Job J;
+        std::memset(&J, 0, sizeof J);
+        J.P = *P;
+        J.geom = s->geom;
+        J.attrs = s->attrs;
+        J.tables = s->tables;
+        J.atlas = s->atlas;
+        J.accum = a->d;
+        J.atlas_side = s->atlas_side;
+        J.first = first;
+        J.count = count;
+        J.tiles_x = (a->w + 7u) / 8u;
+        J.n_items = 64u * J.tiles_x * ((a->h + 7u) / 8u);
+        EventPair ep{};
+        bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
+        if (timed) (void)hipEventRecord(ep.a, c->stream);
+        int rc = wf_run(c, {J});
+        if (timed) {
+            (void)hipEventRecord(ep.b, c->stream);
+            c->events.push_back(ep);
+            c->launches++;
+        }
+        return rc;
+    }
     uint64_t pixels = (uint64_t)a->w * a->h;
     uint64_t per = std::max<uint64_t>(1, c->tuning.max_paths / std::max<uint64_t>(pixels, 1));
     uint32_t block = c->tuning.block;
@@ -476,6 +652,38 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         c->d_counters = (uint32_t *)p;
         c->job_cap = cap;
     }
+    if (c->tuning.kernel == 7) {
+        std::vector<oxd::Job> jobs;
+        for (size_t i = 0; i < n; i++) {
+            const oxpt_batch_item &it = items[i];
+            if (it.count == 0) continue;
+            oxd::Job J;
+            std::memset(&J, 0, sizeof J);
+            J.P = it.params;
+            J.geom = it.scene->geom;
+            J.attrs = it.scene->attrs;
+            J.tables = it.scene->tables;
+            J.atlas = it.scene->atlas;
+            J.accum = it.accum->d;
+            J.atlas_side = it.scene->atlas_side;
+            J.first = it.first;
+            J.count = it.count;
+            J.tiles_x = (it.accum->w + 7u) / 8u;
+            J.n_items = 64u * J.tiles_x * ((it.accum->h + 7u) / 8u);
+            jobs.push_back(J);
+        }
+        if (jobs.empty()) return OXPT_OK;
+        EventPair ep{};
+        bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
+        if (timed) (void)hipEventRecord(ep.a, c->stream);
+        int rc = wf_run(c, jobs);
+        if (timed) {
+            (void)hipEventRecord(ep.b, c->stream);
+            c->events.push_back(ep);
+            c->launches++;
+        }
+        return rc;
+    }
     // Rounds: each round adds at most `spd` samples to every item that still
     // has samples left, so one launch stays under max_paths paths.
     uint64_t total = 0;
@@ -529,7 +737,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
         if (c->use_graph) {
-            if (c->graph && (c->graph_block != block || c->graph_grid != grid)) {
+            if (c->graph && (c->graph_block != block || c->graph_grid != grid || c->graph_kernel != c->tuning.kernel)) {
                 (void)hipGraphExecDestroy(c->graph);
                 c->graph = nullptr;
             }
@@ -539,7 +747,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
                 if (e == hipSuccess) {
-                    hipError_t le = oxd::launch_trace_batch(args, block, grid, c->stream);
+                    hipError_t le = oxd::launch_trace_batch(args, c->tuning.kernel, block, grid, c->stream);
                     e = hipStreamEndCapture(c->stream, &g);
                     if (e == hipSuccess) e = le;
                 }
@@ -548,11 +756,12 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 if (e != hipSuccess) return hip_fail(c, "batch graph capture", e);
                 c->graph_block = block;
                 c->graph_grid = grid;
+                c->graph_kernel = c->tuning.kernel;
             }
             e = hipGraphLaunch(c->graph, c->stream);
             c->graph_launches++;
         } else {
-            e = oxd::launch_trace_batch(args, block, grid, c->stream);
+            e = oxd::launch_trace_batch(args, c->tuning.kernel, block, grid, c->stream);
         }
         if (e != hipSuccess) return hip_fail(c, "batch launch", e);
         if (timed) {

@@ -127,7 +127,8 @@ class COptions(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("block", u32), ("waves_per_cu", u32), ("max_paths", u32), ("kernel", u32)]
+    _fields_ = [("block", u32), ("waves_per_cu", u32), ("max_paths", u32), ("kernel", u32), ("wf_variant", u32),
+                ("reserved", u32 * 3)]
 
 
 class BatchItem(C.Structure):
@@ -572,8 +573,8 @@ class GpuPathTracer:
         lib.oxpt_ctx_timing(self.ctx, 1 if reset else 0, C.byref(n), C.byref(ms))
         return n.value, ms.value
 
-    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel)
+    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant)
         rc = lib.oxpt_ctx_set_tuning(self.ctx, C.byref(t))
         if rc:
             _raise(rc, lib.oxpt_last_error(self.ctx).decode())
@@ -661,8 +662,8 @@ class Device:
         lib.oxpt_ctx_timing(self.h, 1 if reset else 0, C.byref(n), C.byref(ms))
         return n.value, ms.value
 
-    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel)
+    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant)
         self._check(lib.oxpt_ctx_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def get_tuning(self) -> Tuning:
