@@ -137,13 +137,17 @@ typedef struct oxpt_tuning {
     uint32_t block;          /* threads per workgroup: 64, 128, 256 */
     uint32_t waves_per_cu;   /* persistent residency: workgroups = CUs * this / waves per WG */
     uint32_t max_paths;      /* paths per launch cap (samples are split, order kept) */
-    uint32_t kernel;         /* 0 = persistent megakernel, 1 = plain one-pixel-per-thread,
-                                2..6 = persistent built for 3, 4, 5, 6, 8 waves per SIMD (batched: 0 = 2 waves),
-                                7 = wavefront (separate shade / extend / occlude kernels) */
+    uint32_t kernel;         /* 0 = persistent megakernel (4 waves per SIMD), 1 = plain one-pixel-per-thread,
+                                2..6 = persistent built for 3, 4, 5, 6, 8 waves per SIMD,
+                                7 = wavefront (separate shade / extend / occlude kernels),
+                                8 = persistent built for 2 waves per SIMD */
     uint32_t wf_variant;     /* wavefront: bits 0-3 traversal form (occupancy x LDS stack depth),
                                 bits 4-7 shade occupancy */
-    uint32_t reserved[3];
+    uint32_t stack_lds;      /* megakernel traversal-stack entries per lane kept in LDS (the rest of the
+                                64 in private memory): 0 = default for the form, 1..64, or OXPT_STACK_PRIVATE */
+    uint32_t reserved[2];
 } oxpt_tuning;
+#define OXPT_STACK_PRIVATE 0xFFFFu
 int oxpt_ctx_set_tuning(oxpt_ctx *ctx, const oxpt_tuning *t);
 int oxpt_ctx_get_tuning(const oxpt_ctx *ctx, oxpt_tuning *t);
 

@@ -26,10 +26,10 @@ __device__ __forceinline__ Scene make_scene(const oxpt_params &P, const float4 *
                                             const uint32_t *tables, const uint16_t *atlas, uint32_t atlas_side,
                                             const float *lds_sheen) {
     Scene S;
-    S.geom = geom;
-    S.attrs = attrs;
-    S.tables = tables;
-    S.atlas = atlas;
+    S.geom = (gf4_ptr)geom;
+    S.attrs = (gf4_ptr)attrs;
+    S.tables = (gu32_ptr)tables;
+    S.atlas = (gu16_ptr)atlas;
     S.atlas_side = atlas_side;
     S.tri_base = P.offs2[2];
     S.node_count = P.env[3];
@@ -128,18 +128,16 @@ __device__ __forceinline__ void init_path(const oxpt_params &P, const uint32_t *
     st.job = 0;
 }
 
-// One iteration: trace the current ray and advance the state machine
+// Advances the state machine past the ray just traced, given its hit
 // (wgsl:1766-2040). Returns true when the path ended; *covered says
-// whether the camera ray hit anything.
-__device__ bool step_path(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol, PathState &st,
-                          bool *covered) {
+// whether the camera ray hit anything. Otherwise st.r_* holds the next ray.
+__device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol, PathState &st,
+                        const Hit &hit, bool *covered) {
     const uint32_t strategy = P.cfg[1];
     const bool has_env = P.env[2] != 0;
     const uint32_t n_elights = P.cfg[3], n_punct = P.cfg[2];
     const float clampv = P.fparams[1];
     *covered = true;
-    Hit hit = traverse(S, make_ray(st.r_o, st.r_d), st.r_tmax, st.path_seed, st.r_id, !st.shadow && st.k == 0,
-                       st.shadow, st.r_skip);
     if (st.shadow) {
         if (hit.slot == NONE) st.radiance = st.radiance + st.pending;
         st.job += 1;
@@ -358,6 +356,17 @@ __device__ bool step_path(const oxpt_params &P, const Scene &S, const uint32_t *
     return false;
 }
 
+// One iteration: trace the current ray and advance the state machine.
+// `lds_stack` / `lds_depth`: this wave's LDS share of the traversal stack
+// (depth 0: the whole stack in private memory).
+__device__ __forceinline__ bool step_path(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol,
+                                          PathState &st, bool *covered, lds_u64 *lds_stack = nullptr,
+                                          uint32_t lds_depth = 0) {
+    Hit hit = traverse<-1>(S, make_ray(st.r_o, st.r_d), st.r_tmax, st.path_seed, st.r_id, !st.shadow && st.k == 0,
+                           st.shadow, st.r_skip, lds_stack, lds_depth);
+    return advance(P, S, lds_sobol, st, hit, covered);
+}
+
 __device__ __forceinline__ void stage_tables(const uint32_t *tables, const oxpt_params &P, uint32_t *lds_sobol,
                                              float *lds_sheen) {
     for (uint32_t i = threadIdx.x; i < OXPT_SOBOL_WORDS; i += blockDim.x) lds_sobol[i] = tables[P.offs0[0] + i];
@@ -375,11 +384,15 @@ __device__ __forceinline__ void item_pixel(uint32_t tiles_x, uint32_t item, uint
 
 // MinWaves > 0 asks the compiler for that many waves per SIMD (fewer
 // VGPRs, more latency hiding, possible spills): an autotuner knob.
+// lds_depth: traversal-stack entries per lane kept in dynamic LDS
+// (blockDim / 64 * lds_depth * 512 bytes), the rest in private memory.
 template <int MinWaves>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves > 0 ? MinWaves : 1)))
-trace_persistent(TraceArgs A) {
+trace_persistent(TraceArgs A, uint32_t lds_depth) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
+    extern __shared__ uint64_t lds_dyn[];
+    lds_u64 *lds_stack = to_lds(lds_dyn) + (threadIdx.x >> 6) * lds_depth * 64u;
     stage_tables(A.tables, A.P, lds_sobol, lds_sheen);
     const Scene S = make_scene(A, lds_sheen);
     const uint32_t lane = __lane_id();
@@ -426,7 +439,7 @@ trace_persistent(TraceArgs A) {
             need_init = false;
         }
         bool covered;
-        if (step_path(A.P, S, lds_sobol, st, &covered)) {
+        if (step_path(A.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) {
             if (covered) {
                 F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
                 acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
@@ -442,8 +455,9 @@ trace_persistent(TraceArgs A) {
     }
 }
 
-// Plain form: one thread per pixel of an 8x8 tile, samples in order.
-__global__ void __launch_bounds__(64) trace_tiles(TraceArgs A) {
+// Plain form: one thread per pixel of an 8x8 tile, samples in order (built
+// for 4 waves per SIMD, like the default persistent form).
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) trace_tiles(TraceArgs A) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
     stage_tables(A.tables, A.P, lds_sobol, lds_sheen);
@@ -477,9 +491,11 @@ __global__ void __launch_bounds__(64) trace_tiles(TraceArgs A) {
 // so every job's image equals its image alone.
 template <int MinWaves>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves)))
-trace_batch(BatchArgs B) {
+trace_batch(BatchArgs B, uint32_t lds_depth) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
+    extern __shared__ uint64_t lds_dyn[];
+    lds_u64 *lds_stack = to_lds(lds_dyn) + (threadIdx.x >> 6) * lds_depth * 64u;
     // Every job carries the same estimator constants (checked on the host).
     const uint32_t n_jobs = min(*B.n_jobs, B.max_jobs);
     if (n_jobs == 0) return;
@@ -539,7 +555,7 @@ trace_batch(BatchArgs B) {
             need_init = false;
         }
         bool covered;
-        if (step_path(J.P, S, lds_sobol, st, &covered)) {
+        if (step_path(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) {
             if (covered) {
                 F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
                 acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
@@ -677,7 +693,11 @@ __global__ void debug_closest_hit(TraceArgs A, uint32_t n, const float *o3, cons
 
 namespace oxd {
 
-hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, hipStream_t s) {
+// Kernel forms (oxpt_tuning.kernel): 0 the default (4 waves per SIMD),
+// 1 plain per-pixel, 2..6 persistent built for 3, 4, 5, 6, 8 waves per SIMD,
+// 8 persistent built for 2 waves per SIMD (256 VGPRs).
+hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, uint32_t lds_depth,
+                        hipStream_t s) {
     if (kernel == 1) {
         uint32_t blocks = args.n_items / 64u;
         hipLaunchKernelGGL(trace_tiles, dim3(blocks), dim3(64), 0, s, args);
@@ -686,28 +706,32 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s, args.work);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
     switch (kernel) {
-        case 2: hipLaunchKernelGGL(trace_persistent<3>, dim3(grid), dim3(block), 0, s, args); break;
-        case 3: hipLaunchKernelGGL(trace_persistent<4>, dim3(grid), dim3(block), 0, s, args); break;
-        case 4: hipLaunchKernelGGL(trace_persistent<5>, dim3(grid), dim3(block), 0, s, args); break;
-        case 5: hipLaunchKernelGGL(trace_persistent<6>, dim3(grid), dim3(block), 0, s, args); break;
-        case 6: hipLaunchKernelGGL(trace_persistent<8>, dim3(grid), dim3(block), 0, s, args); break;
-        default: hipLaunchKernelGGL(trace_persistent<0>, dim3(grid), dim3(block), 0, s, args); break;
+        case 2: hipLaunchKernelGGL(trace_persistent<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 4: hipLaunchKernelGGL(trace_persistent<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 5: hipLaunchKernelGGL(trace_persistent<6>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 6: hipLaunchKernelGGL(trace_persistent<8>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 8: hipLaunchKernelGGL(trace_persistent<2>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        default: hipLaunchKernelGGL(trace_persistent<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, hipStream_t s) {
+// Batched forms: as above; kernel 8 is the 2-waves-per-SIMD build.
+hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid,
+                              uint32_t lds_depth, hipStream_t s) {
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, s, args.counters, args.n_jobs, args.max_jobs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
     switch (kernel) {
-        case 2: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), 0, s, args); break;
-        case 3: hipLaunchKernelGGL(trace_batch<4>, dim3(grid), dim3(block), 0, s, args); break;
-        case 4: hipLaunchKernelGGL(trace_batch<5>, dim3(grid), dim3(block), 0, s, args); break;
-        case 5: hipLaunchKernelGGL(trace_batch<6>, dim3(grid), dim3(block), 0, s, args); break;
-        case 6: hipLaunchKernelGGL(trace_batch<8>, dim3(grid), dim3(block), 0, s, args); break;
-        default: hipLaunchKernelGGL(trace_batch<2>, dim3(grid), dim3(block), 0, s, args); break;
+        case 2: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 4: hipLaunchKernelGGL(trace_batch<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 5: hipLaunchKernelGGL(trace_batch<6>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 6: hipLaunchKernelGGL(trace_batch<8>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 8: hipLaunchKernelGGL(trace_batch<2>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        default: hipLaunchKernelGGL(trace_batch<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
     }
     return hipGetLastError();
 }

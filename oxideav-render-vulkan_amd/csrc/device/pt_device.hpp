@@ -120,11 +120,33 @@ __device__ __forceinline__ float coin(uint32_t path_seed, uint32_t ray, uint32_t
 
 // ---- Scene view --------------------------------------------------------
 
+// Scene buffers are typed as global-memory (address_space(1)) pointers so
+// every access compiles to global_load even when the pointer itself was
+// read from memory (a job table) rather than from the kernel arguments;
+// generic pointers loaded from memory would otherwise be accessed with
+// slower flat instructions.
+#define OXD_GLOBAL __attribute__((address_space(1)))
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef const OXD_GLOBAL f32x4_t *gf4_ptr;
+typedef const OXD_GLOBAL uint32_t *gu32_ptr;
+typedef const OXD_GLOBAL uint16_t *gu16_ptr;
+
+__device__ __forceinline__ float4 ldg4(gf4_ptr p, uint32_t i) {
+    f32x4_t v = p[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 ldgu4(gu32_ptr p) {
+    u32x4_t v = *(const OXD_GLOBAL u32x4_t *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 struct Scene {
-    const float4 *__restrict__ geom;
-    const float4 *__restrict__ attrs;
-    const uint32_t *__restrict__ tables;
-    const uint16_t *__restrict__ atlas;
+    gf4_ptr geom;
+    gf4_ptr attrs;
+    gu32_ptr tables;
+    gu16_ptr atlas;
     uint32_t atlas_side;
     uint32_t tri_base;    // P.offs2.z
     uint32_t node_count;  // P.env.w
@@ -143,13 +165,13 @@ __device__ __forceinline__ F3 tv3(const Scene &S, uint32_t i) { return {tf(S, i)
 __device__ __forceinline__ float4 tv4(const Scene &S, uint32_t i) {
     return make_float4(tf(S, i), tf(S, i + 1), tf(S, i + 2), tf(S, i + 3));
 }
-__device__ __forceinline__ float4 tri_word(const Scene &S, uint32_t slot, uint32_t v) { return S.geom[S.tri_base + 3u * slot + v]; }
+__device__ __forceinline__ float4 tri_word(const Scene &S, uint32_t slot, uint32_t v) { return ldg4(S.geom, S.tri_base + 3u * slot + v); }
 __device__ __forceinline__ F3 tri_pos(const Scene &S, uint32_t slot, uint32_t v) { return xyz(tri_word(S, slot, v)); }
 __device__ __forceinline__ uint32_t tri_global(const Scene &S, uint32_t slot) { return fbits(tri_word(S, slot, 0).w); }
 __device__ __forceinline__ uint32_t tri_light(const Scene &S, uint32_t slot) { return fbits(tri_word(S, slot, 1).w); }
 __device__ __forceinline__ uint32_t tri_material(const Scene &S, uint32_t slot) { return fbits(tri_word(S, slot, 2).w); }
-__device__ __forceinline__ float4 attr(const Scene &S, uint32_t slot, uint32_t k) { return S.attrs[slot * 12u + k]; }
-__device__ __forceinline__ uint32_t attr_flags(const Scene &S, uint32_t slot) { return fbits(S.attrs[slot * 12u].w); }
+__device__ __forceinline__ float4 attr(const Scene &S, uint32_t slot, uint32_t k) { return ldg4(S.attrs, slot * 12u + k); }
+__device__ __forceinline__ uint32_t attr_flags(const Scene &S, uint32_t slot) { return fbits(ldg4(S.attrs, slot * 12u).w); }
 __device__ __forceinline__ F3 interp3a(const Scene &S, uint32_t slot, uint32_t k, F3 b) {
     return xyz(attr(S, slot, k)) * b.x + xyz(attr(S, slot, k + 1)) * b.y + xyz(attr(S, slot, k + 2)) * b.z;
 }
@@ -179,7 +201,7 @@ __device__ __forceinline__ Level tex_level(const Scene &S, uint32_t desc, uint32
     uint32_t skip = S.tables[desc + 4], kept = S.tables[desc + 5];
     int32_t li = (int32_t)level - (int32_t)skip;
     uint32_t l = min((uint32_t)max(li, 0), kept - 1u);
-    uint4 r = *reinterpret_cast<const uint4 *>(S.tables + desc + 8u + 4u * l);
+    uint4 r = ldgu4(S.tables + desc + 8u + 4u * l);
     return {(int32_t)r.x, (int32_t)r.y, (int32_t)r.z, (int32_t)(r.w & 0xffffu), (int32_t)(r.w >> 16u)};
 }
 // binary16 bits -> f32 (exact).
@@ -198,7 +220,7 @@ __device__ __forceinline__ float4 fetch(const Scene &S, const Level &lv, int32_t
     int32_t xi = wrap_index(x, lv.w, ws), yi = wrap_index(y, lv.h, wt);
     uint64_t side = S.atlas_side;
     uint64_t at = (((uint64_t)lv.layer * side + (uint64_t)(lv.y + yi)) * side + (uint64_t)(lv.x + xi));
-    uint2 raw = reinterpret_cast<const uint2 *>(S.atlas)[at];
+    u32x2_t raw = ((const OXD_GLOBAL u32x2_t *)S.atlas)[at];
     return make_float4(h2f(raw.x & 0xffffu), h2f(raw.x >> 16), h2f(raw.y & 0xffffu), h2f(raw.y >> 16));
 }
 __device__ float4 filter_level(const Scene &S, const Level &lv, float2 uv, bool linear, uint32_t ws, uint32_t wt) {
@@ -385,31 +407,38 @@ __device__ __forceinline__ bool accept(const Scene &S, uint32_t slot, F3 b, bool
 }
 
 // Traversal stack: 64 entries (the reference's depth, wgsl:661-662), each
-// (t_far bits << 32 | node). With LdsDepth > 0 the top LdsDepth entries of
-// every lane live in LDS — laid out [entry][lane] in the wave's own slice so
-// lanes at the same depth touch consecutive words — and deeper entries go to
-// private memory; with LdsDepth == 0 the whole stack is private.
-template <int LdsDepth>
-struct TravStack {
-    uint64_t *lds;  // this wave's slice: LdsDepth * 64 entries (unused when 0)
-    uint32_t lane;
-    uint64_t priv[STACK_DEPTH - LdsDepth];
-    __device__ __forceinline__ void put(uint32_t i, uint64_t v) {
-        if (LdsDepth > 0 && i < (uint32_t)LdsDepth) lds[i * 64u + lane] = v;
-        else priv[i - LdsDepth] = v;
-    }
-    __device__ __forceinline__ uint64_t get(uint32_t i) const {
-        if (LdsDepth > 0 && i < (uint32_t)LdsDepth) return lds[i * 64u + lane];
-        return priv[i - LdsDepth];
-    }
-};
+// (t_far bits << 32 | node). The bottom `lds_depth` entries of every lane
+// live in LDS, laid out [entry][lane] in the wave's own slice so lanes at
+// the same depth touch consecutive words; deeper entries go to private
+// memory. LdsDepth > 0 fixes that share at compile time, LdsDepth < 0 takes
+// it from the run-time `lds_depth` (0..64), LdsDepth == 0 keeps the whole
+// stack private. The LDS pointer is typed address_space(3) so its accesses
+// compile to ds_read / ds_write rather than flat accesses.
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
 
-// Ordered while-while walk (wgsl:651-735): near child first, far child
-// pushed with its entry distance, pops culled by t_max.
+__device__ __forceinline__ lds_u64 *to_lds(uint64_t *p) { return (lds_u64 *)p; }
+
+// Stack entry: bit 63 clear -> bits 62..32 are |t_far| and the low word is
+// the node record itself (first << 5 | count, count <= 30, first < 2^27),
+// so a pop needs no memory access; bit 63 set -> the low word is the node
+// index and the record is read back (leaves past 30 slots or indices past
+// 2^27, which only very large or degenerate scenes produce). t_far >= 0, so
+// dropping its sign bit (+0 for -0) leaves every comparison unchanged.
+__device__ __forceinline__ uint64_t stack_entry(float t_far, uint32_t first, uint32_t count, uint32_t node) {
+    const uint64_t t = (uint64_t)(fbits(t_far) & 0x7fffffffu) << 32;
+    if (count <= 30u && first < (1u << 27)) return t | (first << 5 | count);
+    return t | (1ull << 63) | node;
+}
+
+// Ordered walk (wgsl:651-735): near child first, far child pushed with its
+// entry distance, pops culled by t_max. The current node's record (first,
+// count) travels with it — read from the parent's child pair on descent,
+// from the stack entry on a pop — so each inner-node step issues only the
+// one 64-byte child-pair load.
 template <int LdsDepth = 0>
 __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                         uint32_t ray_id, bool primary, bool any, uint32_t skip,
-                                        uint64_t *lds_stack = nullptr) {
+                                        lds_u64 *lds_stack = nullptr, uint32_t lds_depth = 0) {
     Hit best;
     best.slot = NONE;
     best.t = 0.0f;
@@ -417,14 +446,19 @@ __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_ma
     best.front = false;
     if (S.node_count == 0 || !r.valid) return best;
     float t_max = t_max_in;
-    if (slab(r, S.geom[0], S.geom[1], t_max) < 0.0f) return best;
-    TravStack<LdsDepth> stack;
-    stack.lds = lds_stack;
-    stack.lane = __lane_id();
-    uint32_t sp = 0, idx = 0;
+    uint32_t first, count;
+    {
+        const float4 lo = ldg4(S.geom, 0), hi = ldg4(S.geom, 1);
+        if (slab(r, lo, hi, t_max) < 0.0f) return best;
+        first = fbits(lo.w);
+        count = fbits(hi.w);
+    }
+    const uint32_t in_lds = LdsDepth > 0 ? (uint32_t)LdsDepth : (LdsDepth < 0 ? lds_depth : 0u);
+    lds_u64 *const lds = lds_stack + __lane_id();
+    uint64_t priv[LdsDepth > 0 ? STACK_DEPTH - LdsDepth : STACK_DEPTH];
+    constexpr uint32_t priv_off = LdsDepth > 0 ? (uint32_t)LdsDepth : 0u;
+    uint32_t sp = 0;
     for (;;) {
-        float4 lo = S.geom[2u * idx], hi = S.geom[2u * idx + 1u];
-        uint32_t count = fbits(hi.w), first = fbits(lo.w);
         bool descended = false;
         if (count > 0) {
             bool done = false;
@@ -451,30 +485,32 @@ __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_ma
             }
             if (done) break;
         } else {
-            uint32_t l = first;
-            const float4 *pair = S.geom + 2u * l;
-            float4 llo = pair[0], lhi = pair[1], rlo = pair[2], rhi = pair[3];
-            float tl = slab(r, llo, lhi, t_max);
-            float tr = slab(r, rlo, rhi, t_max);
+            const uint32_t l = first;
+            const float4 llo = ldg4(S.geom, 2u * l), lhi = ldg4(S.geom, 2u * l + 1u),
+                         rlo = ldg4(S.geom, 2u * l + 2u), rhi = ldg4(S.geom, 2u * l + 3u);
+            const float tl = slab(r, llo, lhi, t_max);
+            const float tr = slab(r, rlo, rhi, t_max);
             if (tl >= 0.0f && tr >= 0.0f) {
-                uint32_t near_ = l, far_ = l + 1u;
-                float t_far = tr;
-                if (tl > tr) {
-                    near_ = l + 1u;
-                    far_ = l;
-                    t_far = tl;
-                }
+                const bool swap = tl > tr;
+                const uint32_t far_ = swap ? l : l + 1u;
+                const float t_far = swap ? tl : tr;
+                const uint32_t far_first = fbits(swap ? llo.w : rlo.w), far_count = fbits(swap ? lhi.w : rhi.w);
                 if (sp < STACK_DEPTH) {
-                    stack.put(sp, ((uint64_t)fbits(t_far) << 32) | far_);
+                    const uint64_t e = stack_entry(t_far, far_first, far_count, far_);
+                    if (sp < in_lds) lds[sp * 64u] = e;
+                    else priv[sp - priv_off] = e;
                     sp++;
                 }
-                idx = near_;
+                first = fbits(swap ? rlo.w : llo.w);
+                count = fbits(swap ? rhi.w : lhi.w);
                 descended = true;
             } else if (tl >= 0.0f) {
-                idx = l;
+                first = fbits(llo.w);
+                count = fbits(lhi.w);
                 descended = true;
             } else if (tr >= 0.0f) {
-                idx = l + 1u;
+                first = fbits(rlo.w);
+                count = fbits(rhi.w);
                 descended = true;
             }
         }
@@ -482,9 +518,16 @@ __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_ma
         bool found = false;
         while (sp != 0) {
             sp--;
-            uint64_t e = stack.get(sp);
-            if (bitsf((uint32_t)(e >> 32)) <= t_max) {
-                idx = (uint32_t)e;
+            const uint64_t e = sp < in_lds ? lds[sp * 64u] : priv[sp - priv_off];
+            if (bitsf((uint32_t)(e >> 32) & 0x7fffffffu) <= t_max) {
+                const uint32_t w = (uint32_t)e;
+                if (e >> 63) {
+                    first = fbits(ldg4(S.geom, 2u * w).w);
+                    count = fbits(ldg4(S.geom, 2u * w + 1u).w);
+                } else {
+                    first = w >> 5;
+                    count = w & 31u;
+                }
                 found = true;
                 break;
             }

@@ -78,7 +78,7 @@ __device__ Mat material(const Scene &S, uint32_t slot, F3 b, F3 sn_in, const Lod
             }
         }
     }
-    const uint32_t *mw = S.tables + o;
+    gu32_ptr mw = S.tables + o;
     Mat r;
     float4 col = mul4(make_float4(bitsf(mw[0]), bitsf(mw[1]), bitsf(mw[2]), bitsf(mw[3])), tex[OXPT_S_BASE]);
     if (attr_flags(S, slot) & OXPT_F_COLORS) col = mul4(col, interp4a(S, slot, 9, b));
@@ -477,8 +477,8 @@ struct LightSample {
 };
 
 __device__ LightSample punctual_sample(const Scene &S, uint32_t j, F3 p) {
-    const uint4 *lw = reinterpret_cast<const uint4 *>(S.tables + S.off_plights + 16u * j);
-    uint4 w0 = lw[0], w1 = lw[1], w2 = lw[2], w3 = lw[3];
+    gu32_ptr lw = S.tables + S.off_plights + 16u * j;
+    uint4 w0 = ldgu4(lw), w1 = ldgu4(lw + 4), w2 = ldgu4(lw + 8), w3 = ldgu4(lw + 12);
     LightSample r;
     r.ok = false;
     r.l = f3(0.0f);

@@ -17,6 +17,33 @@
 
 namespace {
 
+// LDS taken by the Sobol' and sheen tables in every trace workgroup.
+constexpr uint32_t kTableLds = 2560;
+
+// Waves per SIMD each megakernel form is built for (oxpt_tuning.kernel).
+uint32_t form_waves(uint32_t kernel) {
+    switch (kernel) {
+        case 2: return 3;
+        case 4: return 5;
+        case 5: return 6;
+        case 6: return 8;
+        case 8: return 2;
+        default: return 4;
+    }
+}
+
+// Traversal-stack entries per lane held in LDS. Default: as many as leave
+// every resident workgroup room on a CU (160 KiB of LDS), at most 16.
+uint32_t stack_lds_entries(const oxpt_tuning &t, uint32_t block) {
+    if (t.stack_lds == OXPT_STACK_PRIVATE) return 0;
+    if (t.stack_lds != 0) return t.stack_lds;
+    const uint32_t waves_cu = 4u * form_waves(t.kernel);
+    const uint32_t wg_cu = std::max(1u, waves_cu / (block / 64u));
+    const uint32_t per_wg = 160u * 1024u / wg_cu;
+    if (per_wg <= kTableLds) return 0;
+    return std::min(16u, (per_wg - kTableLds) / ((block / 64u) * 512u));
+}
+
 // Device opening is serialised process-wide (src/gpu.rs:978-992).
 std::mutex g_device_init;
 
@@ -33,14 +60,14 @@ struct oxpt_ctx {
     std::string summary;
     uint64_t budget = 0, in_use = 0;
     uint32_t n_cu = 0;
-    oxpt_tuning tuning{256, 32, 1u << 27, 0, 0, {0, 0, 0}};
+    oxpt_tuning tuning{256, 32, 1u << 27, 0, 0, 0, {0, 0}};
     uint32_t *d_work = nullptr;
     oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[job_cap]
     uint32_t *d_counters = nullptr;  // job_cap counters + 1 count word
     size_t job_cap = 0;
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;  // captured batched step (reset + trace)
-    uint32_t graph_block = 0, graph_grid = 0, graph_kernel = 0;
+    uint32_t graph_block = 0, graph_grid = 0, graph_kernel = 0, graph_lds = 0;
     uint64_t graph_launches = 0;
     void *d_out = nullptr;
     size_t d_out_bytes = 0;
@@ -385,8 +412,15 @@ int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
     }
     if (t->waves_per_cu) c->tuning.waves_per_cu = std::min(t->waves_per_cu, 64u);
     if (t->max_paths) c->tuning.max_paths = t->max_paths;
+    if (t->kernel > 8) return fail(c, OXPT_E_ARG, "tuning: kernel must be 0..8");
+    if (t->stack_lds != OXPT_STACK_PRIVATE && t->stack_lds > 64)
+        return fail(c, OXPT_E_ARG, "tuning: stack_lds must be 0 (default), 1..64 or OXPT_STACK_PRIVATE");
+    if (t->stack_lds != OXPT_STACK_PRIVATE && t->stack_lds != 0 &&
+        (c->tuning.block / 64u) * t->stack_lds * 512u + kTableLds > 65536u)
+        return fail(c, OXPT_E_ARG, "tuning: stack_lds does not fit in 64 KiB of LDS per workgroup");
     c->tuning.kernel = t->kernel;
     c->tuning.wf_variant = t->wf_variant;
+    c->tuning.stack_lds = t->stack_lds;
     return OXPT_OK;
 }
 int oxpt_ctx_set_graph(oxpt_ctx *c, int enable) {
@@ -587,6 +621,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     uint64_t per = std::max<uint64_t>(1, c->tuning.max_paths / std::max<uint64_t>(pixels, 1));
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
+    const uint32_t lds = stack_lds_entries(c->tuning, block);
     uint64_t end = (uint64_t)first + count;
     for (uint64_t f = first; f < end;) {
         uint32_t n = (uint32_t)std::min<uint64_t>(per, end - f);
@@ -596,7 +631,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
-        hipError_t e = oxd::launch_trace(args, c->tuning.kernel, block, grid, c->stream);
+        hipError_t e = oxd::launch_trace(args, c->tuning.kernel, block, grid, lds, c->stream);
         if (e != hipSuccess) return hip_fail(c, "trace launch", e);
         if (timed) {
             (void)hipEventRecord(ep.b, c->stream);
@@ -695,6 +730,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
     uint32_t spd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_count, c->tuning.max_paths / std::max<uint64_t>(total, 1)));
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
+    const uint32_t lds = stack_lds_entries(c->tuning, block);
     std::vector<oxd::Job> jobs;
     std::vector<double> work;
     for (uint32_t done = 0; done < max_count; done += spd) {
@@ -737,7 +773,8 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
         if (c->use_graph) {
-            if (c->graph && (c->graph_block != block || c->graph_grid != grid || c->graph_kernel != c->tuning.kernel)) {
+            if (c->graph && (c->graph_block != block || c->graph_grid != grid || c->graph_kernel != c->tuning.kernel ||
+                             c->graph_lds != lds)) {
                 (void)hipGraphExecDestroy(c->graph);
                 c->graph = nullptr;
             }
@@ -747,7 +784,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
                 if (e == hipSuccess) {
-                    hipError_t le = oxd::launch_trace_batch(args, c->tuning.kernel, block, grid, c->stream);
+                    hipError_t le = oxd::launch_trace_batch(args, c->tuning.kernel, block, grid, lds, c->stream);
                     e = hipStreamEndCapture(c->stream, &g);
                     if (e == hipSuccess) e = le;
                 }
@@ -757,11 +794,12 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 c->graph_block = block;
                 c->graph_grid = grid;
                 c->graph_kernel = c->tuning.kernel;
+                c->graph_lds = lds;
             }
             e = hipGraphLaunch(c->graph, c->stream);
             c->graph_launches++;
         } else {
-            e = oxd::launch_trace_batch(args, c->tuning.kernel, block, grid, c->stream);
+            e = oxd::launch_trace_batch(args, c->tuning.kernel, block, grid, lds, c->stream);
         }
         if (e != hipSuccess) return hip_fail(c, "batch launch", e);
         if (timed) {

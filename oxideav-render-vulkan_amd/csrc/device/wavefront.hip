@@ -38,10 +38,10 @@ __device__ __forceinline__ F3 clamp_c(float cl, F3 c) {
 
 __device__ __forceinline__ Scene job_scene(const Job &J, const float *lds_sheen) {
     Scene S;
-    S.geom = J.geom;
-    S.attrs = J.attrs;
-    S.tables = J.tables;
-    S.atlas = J.atlas;
+    S.geom = (gf4_ptr)J.geom;
+    S.attrs = (gf4_ptr)J.attrs;
+    S.tables = (gu32_ptr)J.tables;
+    S.atlas = (gu16_ptr)J.atlas;
     S.atlas_side = J.atlas_side;
     S.tri_base = J.P.offs2[2];
     S.node_count = J.P.env[3];
@@ -458,7 +458,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves)
     Scene S = job_scene(J, nullptr);
     Ray r = make_ray(F3{st.o_x, st.o_y, st.o_z}, F3{st.d_x, st.d_y, st.d_z});
     Hit h = traverse<LdsDepth>(S, r, INF, st.path_seed, st.k << 16u, st.k == 0, false, NONE,
-                               lds_stack + (threadIdx.x >> 6) * 64u * (LdsDepth > 0 ? LdsDepth : 0));
+                               to_lds(lds_stack) + (threadIdx.x >> 6) * 64u * (LdsDepth > 0 ? LdsDepth : 0));
     HitRec hr;
     hr.slot = h.slot == NONE ? NONE : (h.slot | (h.front ? 0x80000000u : 0u));
     hr.t = h.t;
@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves)
     ShadowRec rec = A.shadows[(size_t)j * A.n_slots + slot];
     Ray r = make_ray(F3{rec.o_tmax.x, rec.o_tmax.y, rec.o_tmax.z}, F3{rec.d_skip.x, rec.d_skip.y, rec.d_skip.z});
     Hit h = traverse<LdsDepth>(S, r, rec.o_tmax.w, st.path_seed, fbits(rec.pending_rid.w), false, true,
-                               fbits(rec.d_skip.w), lds_stack + (threadIdx.x >> 6) * 64u * (LdsDepth > 0 ? LdsDepth : 0));
+                               fbits(rec.d_skip.w), to_lds(lds_stack) + (threadIdx.x >> 6) * 64u * (LdsDepth > 0 ? LdsDepth : 0));
     A.occluded[(size_t)j * A.n_slots + slot] = h.slot != NONE ? 1u : 0u;
 }
 

@@ -215,6 +215,10 @@ uint32_t build_bvh(const Soup &soup, std::vector<uint32_t> &node_words, std::vec
     for (size_t i = 0; i < n; i++) order[i] = (uint32_t)i;
     Builder bld(tb, cen, order);
     bld.nodes.reserve(2 * n / kMaxLeaf + 8);
+    // Root, then one unused node, so every child pair starts at an even
+    // index: a pair is 64 bytes, and 64-byte aligned it never straddles a
+    // 128-byte cache line.
+    bld.nodes.emplace_back();
     bld.nodes.emplace_back();
     bld.build(0, 0, (uint32_t)n, 0);
     node_words.resize(bld.nodes.size() * 8);

@@ -143,6 +143,30 @@ def test_plain_kernel_matches_persistent(gpu):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_kernel_forms_and_stack_split_bitwise(gpu):
+    """Every occupancy form and every LDS/private split of the traversal stack
+    (including a 1-entry LDS share, so deep pushes land in private memory)
+    walks the same nodes in the same order: identical accumulators."""
+    scene = small_terrain()
+    opts = oxpt.RenderOptions(width=96, height=64, samples_per_pixel=4, max_bounces=6)
+    gpu.set_environment(None)
+    gpu.set_tuning(kernel=8, stack_lds=oxpt.STACK_PRIVATE)
+    ref = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+    for kernel, lds in ((0, 0), (0, 1), (0, 5), (2, 24), (3, oxpt.STACK_PRIVATE), (4, 0), (5, 3), (6, 0), (8, 16)):
+        gpu.set_tuning(kernel=kernel, stack_lds=lds)
+        got = gpu.render_hdr(scene, opts).view(np.uint32)
+        assert np.array_equal(got, ref), f"kernel {kernel} stack_lds {lds}"
+    gpu.set_tuning(kernel=0)
+
+
+def test_tuning_rejects_bad_stack_split(gpu):
+    with pytest.raises(oxpt.Error):
+        gpu.set_tuning(stack_lds=65)
+    with pytest.raises(oxpt.Error):
+        gpu.set_tuning(kernel=9)
+    gpu.set_tuning(kernel=0)
+
+
 # ---------------------------------------------------------------------------
 # The reference's parity scenes (tests/pathtrace.rs:319-418): every lobe,
 # punctual lights, alpha modes, textures with mips, direct-only.

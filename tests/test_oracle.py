@@ -177,10 +177,19 @@ def test_bvh_leaf_slots_cover_every_triangle_once():
     for f, c in zip(first[leaves], count[leaves]):
         covered[f: f + c] += 1
     assert (covered == 1).all()
-    # Children boxes lie inside their parent's box.
-    for i in np.nonzero(~leaves)[0]:
+    # Walking from the root: children boxes lie inside their parent's box,
+    # child pairs start at even indices (64-byte aligned), every leaf is reached.
+    stack, reached = [0], 0
+    while stack:
+        i = stack.pop()
+        if count[i] > 0:
+            reached += 1
+            continue
+        assert first[i] % 2 == 0
         for ch in (first[i], first[i] + 1):
             assert np.all(nodes[ch, :3] >= nodes[i, :3]) and np.all(nodes[ch, 4:7] <= nodes[i, 4:7])
+            stack.append(ch)
+    assert reached == int(leaves.sum())
     # Global ids in the triangle words are a permutation.
     tris = pk.geom[8 * n_nodes:].reshape(-1, 12)
     gids = tris[:, 3].view(np.uint32)
