@@ -42,7 +42,7 @@ typedef struct oxpt_params {
     uint32_t env[4];   /* env width, env height, has env, node count */
     uint32_t offs0[4]; /* table word offsets: sobol, sheen, materials, punctual lights */
     uint32_t offs1[4]; /* emissive lights, unused, env marginal, env conditional */
-    uint32_t offs2[4]; /* env mass, env radiance, first triangle float4 in geom, unused */
+    uint32_t offs2[4]; /* env mass, env radiance, first triangle float4 in geom, first 4-wide node float4 (0: none) */
     float fparams[4];  /* ambient, clamp, P_env, unused */
 } oxpt_params;
 

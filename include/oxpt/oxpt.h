@@ -137,15 +137,19 @@ typedef struct oxpt_tuning {
     uint32_t block;          /* threads per workgroup: 64, 128, 256 */
     uint32_t waves_per_cu;   /* persistent residency: workgroups = CUs * this / waves per WG */
     uint32_t max_paths;      /* paths per launch cap (samples are split, order kept) */
-    uint32_t kernel;         /* 0 = persistent megakernel (4 waves per SIMD), 1 = plain one-pixel-per-thread,
-                                2..6 = persistent built for 3, 4, 5, 6, 8 waves per SIMD,
-                                7 = wavefront (separate shade / extend / occlude kernels),
+    uint32_t kernel;         /* 0 = default: oxpt_refine runs the persistent megakernel at 5 waves per SIMD;
+                                oxpt_refine_batch runs the wavefront engine, or with graph replay enabled
+                                the batched megakernel at 3 waves per SIMD;
+                                1 = plain one-pixel-per-thread, 2..6 = persistent built for 3, 4, 5, 6, 8
+                                waves per SIMD, 7 = wavefront (separate shade / extend / occlude kernels),
                                 8 = persistent built for 2 waves per SIMD */
-    uint32_t wf_variant;     /* wavefront: bits 0-3 traversal form (occupancy x LDS stack depth),
-                                bits 4-7 shade occupancy */
-    uint32_t stack_lds;      /* megakernel traversal-stack entries per lane kept in LDS (the rest of the
-                                64 in private memory): 0 = default for the form, 1..64, or OXPT_STACK_PRIVATE */
-    uint32_t reserved[2];
+    uint32_t wf_variant;     /* wavefront: bits 0-3 traversal form (0 = 6 waves / 8 stack entries in LDS,
+                                1 = 5 waves / private stack, 2 = 8 / 8, 3 = 5 / 16, 4 = 6 / private),
+                                bits 4-7 shade occupancy (0 = 3 waves, 1 = 2, 2 = 4) */
+    uint32_t stack_lds;      /* megakernel traversal-stack entries per lane kept in LDS (the rest in private
+                                memory): 0 = default for the form, 1..64, or OXPT_STACK_PRIVATE */
+    uint32_t bvh;            /* 0 = walk the scene's 4-wide tree when it has one, 2 = the binary tree */
+    uint32_t reserved;
 } oxpt_tuning;
 #define OXPT_STACK_PRIVATE 0xFFFFu
 int oxpt_ctx_set_tuning(oxpt_ctx *ctx, const oxpt_tuning *t);

@@ -32,6 +32,7 @@ __device__ __forceinline__ Scene make_scene(const oxpt_params &P, const float4 *
     S.atlas = (gu16_ptr)atlas;
     S.atlas_side = atlas_side;
     S.tri_base = P.offs2[2];
+    S.wide_base = P.offs2[3];
     S.node_count = P.env[3];
     S.off_mat = P.offs0[2];
     S.off_plights = P.offs0[3];
@@ -693,7 +694,7 @@ __global__ void debug_closest_hit(TraceArgs A, uint32_t n, const float *o3, cons
 
 namespace oxd {
 
-// Kernel forms (oxpt_tuning.kernel): 0 the default (4 waves per SIMD),
+// Kernel forms (oxpt_tuning.kernel): 0 the default (5 waves per SIMD),
 // 1 plain per-pixel, 2..6 persistent built for 3, 4, 5, 6, 8 waves per SIMD,
 // 8 persistent built for 2 waves per SIMD (256 VGPRs).
 hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, uint32_t lds_depth,
@@ -709,16 +710,16 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
     switch (kernel) {
         case 2: hipLaunchKernelGGL(trace_persistent<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        case 4: hipLaunchKernelGGL(trace_persistent<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 3: hipLaunchKernelGGL(trace_persistent<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 5: hipLaunchKernelGGL(trace_persistent<6>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 6: hipLaunchKernelGGL(trace_persistent<8>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 8: hipLaunchKernelGGL(trace_persistent<2>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        default: hipLaunchKernelGGL(trace_persistent<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        default: hipLaunchKernelGGL(trace_persistent<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
     }
     return hipGetLastError();
 }
 
-// Batched forms: as above; kernel 8 is the 2-waves-per-SIMD build.
+// Batched forms: as above, except that the default is the 3-waves build.
 hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid,
                               uint32_t lds_depth, hipStream_t s) {
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, s, args.counters, args.n_jobs, args.max_jobs);
@@ -726,12 +727,12 @@ hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t b
     if (e != hipSuccess) return e;
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
     switch (kernel) {
-        case 2: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 3: hipLaunchKernelGGL(trace_batch<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 4: hipLaunchKernelGGL(trace_batch<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 5: hipLaunchKernelGGL(trace_batch<6>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 6: hipLaunchKernelGGL(trace_batch<8>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 8: hipLaunchKernelGGL(trace_batch<2>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        default: hipLaunchKernelGGL(trace_batch<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        default: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
     }
     return hipGetLastError();
 }

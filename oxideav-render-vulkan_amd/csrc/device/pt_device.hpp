@@ -149,6 +149,7 @@ struct Scene {
     gu16_ptr atlas;
     uint32_t atlas_side;
     uint32_t tri_base;    // P.offs2.z
+    uint32_t wide_base;   // P.offs2.w: first 4-wide node float4 (0: walk the binary tree)
     uint32_t node_count;  // P.env.w
     uint32_t off_mat;     // P.offs0.z
     uint32_t off_plights; // P.offs0.w
@@ -435,10 +436,10 @@ __device__ __forceinline__ uint64_t stack_entry(float t_far, uint32_t first, uin
 // count) travels with it — read from the parent's child pair on descent,
 // from the stack entry on a pop — so each inner-node step issues only the
 // one 64-byte child-pair load.
-template <int LdsDepth = 0>
-__device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
-                                        uint32_t ray_id, bool primary, bool any, uint32_t skip,
-                                        lds_u64 *lds_stack = nullptr, uint32_t lds_depth = 0) {
+template <int LdsDepth>
+__device__ __forceinline__ Hit traverse2(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
+                                         uint32_t ray_id, bool primary, bool any, uint32_t skip, lds_u64 *lds_stack,
+                                         uint32_t lds_depth) {
     Hit best;
     best.slot = NONE;
     best.t = 0.0f;
@@ -535,6 +536,143 @@ __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_ma
         if (!found) break;
     }
     return best;
+}
+
+// Leaf: test slots [first, first + count) in order (wgsl:683-699); returns
+// true when an any-hit query is answered.
+__device__ __forceinline__ bool leaf_hits(const Scene &S, const Ray &r, uint32_t first, uint32_t count, float &t_max,
+                                          Hit &best, uint32_t path_seed, uint32_t ray_id, bool primary, bool any,
+                                          uint32_t skip) {
+    for (uint32_t slot = first; slot < first + count; slot++) {
+        float4 w0 = tri_word(S, slot, 0), w1 = tri_word(S, slot, 1), w2 = tri_word(S, slot, 2);
+        float t;
+        F3 b;
+        bool front;
+        if (!intersect_tri(r, xyz(w0), xyz(w1), xyz(w2), t_max, t, b, front)) continue;
+        uint32_t gid = fbits(w0.w);
+        if (gid == skip) continue;
+        uint32_t mat = fbits(w2.w);
+        uint32_t fl = mat_flags(S, mat);
+        if (!accept(S, slot, b, front, mat, fl, path_seed, ray_id, primary, gid)) continue;
+        best.slot = slot;
+        best.t = t;
+        best.b = b;
+        best.front = front;
+        if (any) return true;
+        t_max = t;
+    }
+    return false;
+}
+
+// Walk of the 4-wide tree (the binary tree's boxes and leaves, two levels
+// per node; see bvh.cpp). Per node: one 128-byte load, four slab tests,
+// the hit children ordered by entry distance (ties by slot), the nearest
+// entered and the others pushed farthest first. Every box on the way to
+// a leaf is the binary walk's box, so the leaves reached and the closest
+// accepted hit are the same; only the order between leaves differs, which
+// decides nothing but exact ties in t. A node has at most 3 pushes, and
+// the builder bounds 3 * depth by WIDE_STACK, so no entry is ever dropped.
+constexpr uint32_t WIDE_STACK = 96;
+constexpr uint32_t WIDE_EMPTY = 0xffffffffu;
+
+template <int LdsDepth>
+__device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
+                                         uint32_t ray_id, bool primary, bool any, uint32_t skip, lds_u64 *lds_stack,
+                                         uint32_t lds_depth) {
+    Hit best;
+    best.slot = NONE;
+    best.t = 0.0f;
+    best.b = f3(0.0f);
+    best.front = false;
+    if (S.node_count == 0 || !r.valid) return best;
+    float t_max = t_max_in;
+    if (slab(r, ldg4(S.geom, 0), ldg4(S.geom, 1), t_max) < 0.0f) return best;
+    const uint32_t in_lds = LdsDepth > 0 ? (uint32_t)LdsDepth : (LdsDepth < 0 ? lds_depth : 0u);
+    lds_u64 *const lds = lds_stack + __lane_id();
+    uint64_t priv[LdsDepth > 0 ? WIDE_STACK - LdsDepth : WIDE_STACK];
+    constexpr uint32_t priv_off = LdsDepth > 0 ? (uint32_t)LdsDepth : 0u;
+    uint32_t sp = 0, first = 0, count = 0;  // wide node 0, inner
+    for (;;) {
+        bool descended = false;
+        if (count > 0) {
+            if (leaf_hits(S, r, first, count, t_max, best, path_seed, ray_id, primary, any, skip)) break;
+        } else {
+            const gf4_ptr nd = S.geom + S.wide_base + 8u * first;
+            const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
+            const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
+            const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
+            // Sort keys: (|t| bits << 32 | child) for hit children, all ones for misses.
+            uint64_t k[4];
+            {
+                const float t0 = cn.x != WIDE_EMPTY ? slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f),
+                                                           make_float4(hx.x, hy.x, hz.x, 0.0f), t_max) : -1.0f;
+                const float t1 = cn.y != WIDE_EMPTY ? slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f),
+                                                           make_float4(hx.y, hy.y, hz.y, 0.0f), t_max) : -1.0f;
+                const float t2 = cn.z != WIDE_EMPTY ? slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f),
+                                                           make_float4(hx.z, hy.z, hz.z, 0.0f), t_max) : -1.0f;
+                const float t3 = cn.w != WIDE_EMPTY ? slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f),
+                                                           make_float4(hx.w, hy.w, hz.w, 0.0f), t_max) : -1.0f;
+                k[0] = t0 >= 0.0f ? ((uint64_t)(fbits(t0) & 0x7fffffffu) << 32) | 0u : ~0ull;
+                k[1] = t1 >= 0.0f ? ((uint64_t)(fbits(t1) & 0x7fffffffu) << 32) | 1u : ~0ull;
+                k[2] = t2 >= 0.0f ? ((uint64_t)(fbits(t2) & 0x7fffffffu) << 32) | 2u : ~0ull;
+                k[3] = t3 >= 0.0f ? ((uint64_t)(fbits(t3) & 0x7fffffffu) << 32) | 3u : ~0ull;
+            }
+#define OXD_CSWAP(a, b)                            \
+    {                                              \
+        const uint64_t lo_ = k[a] < k[b] ? k[a] : k[b]; \
+        const uint64_t hi_ = k[a] < k[b] ? k[b] : k[a]; \
+        k[a] = lo_;                                \
+        k[b] = hi_;                                \
+    }
+            OXD_CSWAP(0, 1) OXD_CSWAP(2, 3) OXD_CSWAP(0, 2) OXD_CSWAP(1, 3) OXD_CSWAP(1, 2)
+#undef OXD_CSWAP
+            if (k[0] != ~0ull) {
+                // Push the farther hit children, farthest first.
+#pragma unroll
+                for (int j = 3; j >= 1; j--) {
+                    if (k[j] == ~0ull) continue;
+                    const uint32_t c = (uint32_t)k[j] & 3u;
+                    const uint32_t cref = c == 0 ? rf.x : c == 1 ? rf.y : c == 2 ? rf.z : rf.w;
+                    const uint32_t ccnt = c == 0 ? cn.x : c == 1 ? cn.y : c == 2 ? cn.z : cn.w;
+                    if (sp < WIDE_STACK) {
+                        const uint64_t e = (k[j] & 0xffffffff00000000ull) | (cref << 5 | ccnt);
+                        if (sp < in_lds) lds[sp * 64u] = e;
+                        else priv[sp - priv_off] = e;
+                        sp++;
+                    }
+                }
+                const uint32_t c = (uint32_t)k[0] & 3u;
+                first = c == 0 ? rf.x : c == 1 ? rf.y : c == 2 ? rf.z : rf.w;
+                count = c == 0 ? cn.x : c == 1 ? cn.y : c == 2 ? cn.z : cn.w;
+                descended = true;
+            }
+        }
+        if (descended) continue;
+        bool found = false;
+        while (sp != 0) {
+            sp--;
+            const uint64_t e = sp < in_lds ? lds[sp * 64u] : priv[sp - priv_off];
+            if (bitsf((uint32_t)(e >> 32)) <= t_max) {
+                first = (uint32_t)e >> 5;
+                count = (uint32_t)e & 31u;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return best;
+}
+
+// Closest (or, with `any`, some) accepted hit of the ray up to t_max_in:
+// the 4-wide walk when the scene carries one, else the binary walk.
+template <int LdsDepth = 0>
+__device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
+                                        uint32_t ray_id, bool primary, bool any, uint32_t skip,
+                                        lds_u64 *lds_stack = nullptr, uint32_t lds_depth = 0) {
+    if (S.wide_base != 0)
+        return traverse4<LdsDepth>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack, lds_depth);
+    return traverse2<LdsDepth>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack, lds_depth);
 }
 
 __device__ __forceinline__ float offset_one(float p, float n) {

@@ -20,10 +20,13 @@ namespace {
 // LDS taken by the Sobol' and sheen tables in every trace workgroup.
 constexpr uint32_t kTableLds = 2560;
 
-// Waves per SIMD each megakernel form is built for (oxpt_tuning.kernel).
-uint32_t form_waves(uint32_t kernel) {
+// Waves per SIMD each megakernel form is built for (oxpt_tuning.kernel;
+// kernel 0 is 5 waves for a single refine and 3 for the batched megakernel).
+uint32_t form_waves(uint32_t kernel, bool batched) {
     switch (kernel) {
+        case 0: return batched ? 3 : 5;
         case 2: return 3;
+        case 3: return 4;
         case 4: return 5;
         case 5: return 6;
         case 6: return 8;
@@ -34,10 +37,10 @@ uint32_t form_waves(uint32_t kernel) {
 
 // Traversal-stack entries per lane held in LDS. Default: as many as leave
 // every resident workgroup room on a CU (160 KiB of LDS), at most 16.
-uint32_t stack_lds_entries(const oxpt_tuning &t, uint32_t block) {
+uint32_t stack_lds_entries(const oxpt_tuning &t, uint32_t block, bool batched) {
     if (t.stack_lds == OXPT_STACK_PRIVATE) return 0;
     if (t.stack_lds != 0) return t.stack_lds;
-    const uint32_t waves_cu = 4u * form_waves(t.kernel);
+    const uint32_t waves_cu = 4u * form_waves(t.kernel, batched);
     const uint32_t wg_cu = std::max(1u, waves_cu / (block / 64u));
     const uint32_t per_wg = 160u * 1024u / wg_cu;
     if (per_wg <= kTableLds) return 0;
@@ -60,7 +63,7 @@ struct oxpt_ctx {
     std::string summary;
     uint64_t budget = 0, in_use = 0;
     uint32_t n_cu = 0;
-    oxpt_tuning tuning{256, 32, 1u << 27, 0, 0, 0, {0, 0}};
+    oxpt_tuning tuning{256, 32, 1u << 27, 0, 0, 0, 0, 0};
     uint32_t *d_work = nullptr;
     oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[job_cap]
     uint32_t *d_counters = nullptr;  // job_cap counters + 1 count word
@@ -102,6 +105,7 @@ struct oxpt_scene {
     uint16_t *atlas = nullptr;
     uint32_t atlas_side = 1, atlas_layers = 1;
     size_t table_words = 0;
+    size_t geom_vec4 = 0;
     size_t bytes = 0;
 };
 
@@ -181,6 +185,7 @@ oxd::TraceArgs trace_args(const oxpt_scene *s, oxpt_accum *acc, const oxpt_param
         a.n_items = 64u * tx * ty;
     }
     a.work = s->ctx->d_work;
+    if (s->ctx->tuning.bvh == 2) a.P.offs2[3] = 0;  // walk the binary tree
     return a;
 }
 
@@ -198,6 +203,8 @@ int check_params(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P) {
             !in(P->offs2[1], 3 * w * h))
             return fail(c, OXPT_E_ARG, "params: environment sections outside the tables");
     }
+    if (P->offs2[3] != 0 && (P->offs2[3] % 8u != 0 || (uint64_t)P->offs2[3] + 8u > s->geom_vec4))
+        return fail(c, OXPT_E_ARG, "params: 4-wide nodes outside the geometry");
     return OXPT_OK;
 }
 
@@ -418,9 +425,11 @@ int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
     if (t->stack_lds != OXPT_STACK_PRIVATE && t->stack_lds != 0 &&
         (c->tuning.block / 64u) * t->stack_lds * 512u + kTableLds > 65536u)
         return fail(c, OXPT_E_ARG, "tuning: stack_lds does not fit in 64 KiB of LDS per workgroup");
+    if (t->bvh != 0 && t->bvh != 2) return fail(c, OXPT_E_ARG, "tuning: bvh must be 0 (4-wide) or 2 (binary)");
     c->tuning.kernel = t->kernel;
     c->tuning.wf_variant = t->wf_variant;
     c->tuning.stack_lds = t->stack_lds;
+    c->tuning.bvh = t->bvh;
     return OXPT_OK;
 }
 int oxpt_ctx_set_graph(oxpt_ctx *c, int enable) {
@@ -446,6 +455,7 @@ int oxpt_scene_upload(oxpt_ctx *c, const oxpt_scene_desc *d, oxpt_scene **out) {
     auto *s = new oxpt_scene();
     s->ctx = c;
     s->table_words = d->table_words;
+    s->geom_vec4 = d->geom_vec4;
     s->atlas_side = std::max(d->atlas_side, 1u);
     s->atlas_layers = std::max(d->atlas_layers, 1u);
     size_t gb = d->geom_vec4 * 16, ab = d->attrs_vec4 * 16, tb = d->table_words * 4;
@@ -596,6 +606,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         oxd::Job J;
         std::memset(&J, 0, sizeof J);
         J.P = *P;
+        if (c->tuning.bvh == 2) J.P.offs2[3] = 0;
         J.geom = s->geom;
         J.attrs = s->attrs;
         J.tables = s->tables;
@@ -621,7 +632,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     uint64_t per = std::max<uint64_t>(1, c->tuning.max_paths / std::max<uint64_t>(pixels, 1));
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
-    const uint32_t lds = stack_lds_entries(c->tuning, block);
+    const uint32_t lds = stack_lds_entries(c->tuning, block, false);
     uint64_t end = (uint64_t)first + count;
     for (uint64_t f = first; f < end;) {
         uint32_t n = (uint32_t)std::min<uint64_t>(per, end - f);
@@ -687,7 +698,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         c->d_counters = (uint32_t *)p;
         c->job_cap = cap;
     }
-    if (c->tuning.kernel == 7) {
+    if (c->tuning.kernel == 7 || (c->tuning.kernel == 0 && !c->use_graph)) {
         std::vector<oxd::Job> jobs;
         for (size_t i = 0; i < n; i++) {
             const oxpt_batch_item &it = items[i];
@@ -695,6 +706,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
             oxd::Job J;
             std::memset(&J, 0, sizeof J);
             J.P = it.params;
+            if (c->tuning.bvh == 2) J.P.offs2[3] = 0;
             J.geom = it.scene->geom;
             J.attrs = it.scene->attrs;
             J.tables = it.scene->tables;
@@ -730,7 +742,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
     uint32_t spd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_count, c->tuning.max_paths / std::max<uint64_t>(total, 1)));
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
-    const uint32_t lds = stack_lds_entries(c->tuning, block);
+    const uint32_t lds = stack_lds_entries(c->tuning, block, true);
     std::vector<oxd::Job> jobs;
     std::vector<double> work;
     for (uint32_t done = 0; done < max_count; done += spd) {
@@ -742,6 +754,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
             oxd::Job J;
             std::memset(&J, 0, sizeof J);
             J.P = it.params;
+            if (c->tuning.bvh == 2) J.P.offs2[3] = 0;
             J.geom = it.scene->geom;
             J.attrs = it.scene->attrs;
             J.tables = it.scene->tables;

@@ -44,6 +44,7 @@ __device__ __forceinline__ Scene job_scene(const Job &J, const float *lds_sheen)
     S.atlas = (gu16_ptr)J.atlas;
     S.atlas_side = J.atlas_side;
     S.tri_base = J.P.offs2[2];
+    S.wide_base = J.P.offs2[3];
     S.node_count = J.P.env[3];
     S.off_mat = J.P.offs0[2];
     S.off_plights = J.P.offs0[3];
@@ -524,22 +525,22 @@ hipError_t wf_launch_clear(const WfArgs &A, hipStream_t s) {
     return hipGetLastError();
 }
 
-// variant: bits 0-3 trace form (0: 5 waves/private stack, 1: 6 waves/8 in
-// LDS, 2: 8 waves/8 in LDS, 3: 5 waves/16 in LDS, 4: 6 waves/private),
-// bits 4-7 shade occupancy (0: 2 waves, 1: 3, 2: 4).
+// variant: bits 0-3 trace form (0: 6 waves/8 stack entries in LDS, 1: 5
+// waves/private stack, 2: 8 waves/8 in LDS, 3: 5 waves/16 in LDS, 4: 6
+// waves/private), bits 4-7 shade occupancy (0: 3 waves, 1: 2, 2: 4).
 hipError_t wf_launch_iteration(const WfArgs &A, uint32_t variant, hipStream_t s) {
     uint32_t blocks = (A.n_slots + 255u) / 256u;
     switch ((variant >> 4) & 15u) {
-        case 1: hipLaunchKernelGGL(wf_shade<3>, dim3(blocks), dim3(256), 0, s, A); break;
+        case 1: hipLaunchKernelGGL(wf_shade<2>, dim3(blocks), dim3(256), 0, s, A); break;
         case 2: hipLaunchKernelGGL(wf_shade<4>, dim3(blocks), dim3(256), 0, s, A); break;
-        default: hipLaunchKernelGGL(wf_shade<2>, dim3(blocks), dim3(256), 0, s, A); break;
+        default: hipLaunchKernelGGL(wf_shade<3>, dim3(blocks), dim3(256), 0, s, A); break;
     }
     uint64_t sh = (uint64_t)A.n_slots * A.max_shadow;
     dim3 sb((uint32_t)((sh + 255u) / 256u));
     switch (variant & 15u) {
         case 1:
-            hipLaunchKernelGGL((wf_extend<6, 8>), dim3(blocks), dim3(256), 0, s, A);
-            if (sh) hipLaunchKernelGGL((wf_occlude<6, 8>), sb, dim3(256), 0, s, A);
+            hipLaunchKernelGGL((wf_extend<5, 0>), dim3(blocks), dim3(256), 0, s, A);
+            if (sh) hipLaunchKernelGGL((wf_occlude<5, 0>), sb, dim3(256), 0, s, A);
             break;
         case 2:
             hipLaunchKernelGGL((wf_extend<8, 8>), dim3(blocks), dim3(256), 0, s, A);
@@ -554,8 +555,8 @@ hipError_t wf_launch_iteration(const WfArgs &A, uint32_t variant, hipStream_t s)
             if (sh) hipLaunchKernelGGL((wf_occlude<6, 0>), sb, dim3(256), 0, s, A);
             break;
         default:
-            hipLaunchKernelGGL((wf_extend<5, 0>), dim3(blocks), dim3(256), 0, s, A);
-            if (sh) hipLaunchKernelGGL((wf_occlude<5, 0>), sb, dim3(256), 0, s, A);
+            hipLaunchKernelGGL((wf_extend<6, 8>), dim3(blocks), dim3(256), 0, s, A);
+            if (sh) hipLaunchKernelGGL((wf_occlude<6, 8>), sb, dim3(256), 0, s, A);
             break;
     }
     return hipGetLastError();

@@ -235,4 +235,120 @@ uint32_t build_bvh(const Soup &soup, std::vector<uint32_t> &node_words, std::vec
     return bld.depth;
 }
 
+
+
+// ---- 4-wide BVH collapsed from the binary tree -------------------------
+//
+// Node = 8 float4 (128 bytes, one cache line): lo.x[4], lo.y[4], lo.z[4],
+// hi.x[4], hi.y[4], hi.z[4], ref[4], count[4]. Child i is an inner wide
+// node (count 0, ref = wide node index), a leaf (count 1..30, ref = first
+// leaf slot) or empty (count 0xffffffff). Child boxes and leaves are the
+// binary tree's, bit for bit, so the slab and triangle tests see the same
+// numbers. Each wide node takes the binary node's two children and
+// repeatedly opens the inner child with the largest surface area until it
+// has 4 children.
+namespace {
+
+constexpr uint32_t kWideLeafMax = 30;
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+struct WideChild {
+    float lo[3], hi[3];
+    uint32_t first, count;  // the binary node's record
+};
+
+struct WideBuilder {
+    const std::vector<uint32_t> &nw;  // binary node words
+    std::vector<float> out;           // 32 words per wide node
+    uint32_t depth = 0;
+    bool ok = true;
+
+    explicit WideBuilder(const std::vector<uint32_t> &n) : nw(n) {}
+
+    WideChild of_bin(uint32_t n) const {
+        WideChild c;
+        for (int k = 0; k < 3; k++) {
+            std::memcpy(&c.lo[k], &nw[8 * (size_t)n + k], 4);
+            std::memcpy(&c.hi[k], &nw[8 * (size_t)n + 4 + k], 4);
+        }
+        c.first = nw[8 * (size_t)n + 3];
+        c.count = nw[8 * (size_t)n + 7];
+        return c;
+    }
+    static float area(const WideChild &c) {
+        float d[3];
+        for (int k = 0; k < 3; k++) d[k] = std::max(c.hi[k] - c.lo[k], 0.0f);
+        return d[0] * d[1] + d[1] * d[2] + d[2] * d[0];
+    }
+    std::vector<WideChild> open(const WideChild &c) const { return {of_bin(c.first), of_bin(c.first + 1)}; }
+
+    // Emits the wide node over the given children; returns its index.
+    uint32_t emit(std::vector<WideChild> kids, uint32_t d) {
+        depth = std::max(depth, d);
+        while (kids.size() < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (size_t i = 0; i < kids.size(); i++) {
+                if (kids[i].count != 0) continue;
+                float a = area(kids[i]);
+                if (a > ba) {
+                    ba = a;
+                    best = (int)i;
+                }
+            }
+            if (best < 0) break;
+            std::vector<WideChild> sub = open(kids[best]);
+            kids.erase(kids.begin() + best);
+            kids.insert(kids.begin() + best, sub.begin(), sub.end());
+        }
+        const uint32_t me = (uint32_t)(out.size() / 32);
+        out.resize(out.size() + 32, 0.0f);
+        uint32_t refs[4], cnts[4];
+        float box[6][4];
+        for (int i = 0; i < 4; i++) {
+            const bool real = i < (int)kids.size();
+            for (int k = 0; k < 3; k++) {
+                box[k][i] = real ? kids[i].lo[k] : INFINITY;
+                box[3 + k][i] = real ? kids[i].hi[k] : -INFINITY;
+            }
+            refs[i] = 0;
+            cnts[i] = kEmpty;
+        }
+        for (int i = 0; i < (int)kids.size(); i++) {
+            if (kids[i].count == 0) {
+                refs[i] = emit(open(kids[i]), d + 1);
+                cnts[i] = 0;
+            } else {
+                refs[i] = kids[i].first;
+                cnts[i] = kids[i].count;
+                if (cnts[i] > kWideLeafMax) ok = false;
+            }
+            if (refs[i] >= (1u << 27)) ok = false;
+        }
+        float *o = &out[32 * (size_t)me];
+        for (int r = 0; r < 6; r++)
+            for (int i = 0; i < 4; i++) o[4 * r + i] = box[r][i];
+        std::memcpy(o + 24, refs, 16);
+        std::memcpy(o + 28, cnts, 16);
+        return me;
+    }
+};
+
+}  // namespace
+
+uint32_t build_wide(const std::vector<uint32_t> &node_words, std::vector<float> &wide) {
+    wide.clear();
+    if (node_words.size() < 8) return 0;
+    WideBuilder wb(node_words);
+    const WideChild root = wb.of_bin(0);
+    std::vector<WideChild> kids;
+    if (root.count == 0) kids = wb.open(root);
+    else kids.push_back(root);
+    wb.emit(kids, 1);
+    // At most three entries per level go on the stack.
+    if (!wb.ok || 3u * wb.depth > kWideStack) return 0;
+    wide.swap(wb.out);
+    return wb.depth;
+}
+
 }  // namespace oxpt

@@ -401,6 +401,17 @@ Packed pack_scene(const HostScene &s) {
     } else {
         fill(0, order.size());
     }
+    // ---- 4-wide nodes after the triangles, 128-byte aligned.
+    if (!order.empty()) {
+        std::vector<float> wide;
+        pk.wide_depth = build_wide(node_words, wide);
+        if (!wide.empty()) {
+            size_t base = (pk.geom.size() / 4 + 7) / 8 * 8;
+            pk.wide_base = (uint32_t)base;
+            pk.geom.resize(4 * base, 0.0f);
+            pk.geom.insert(pk.geom.end(), wide.begin(), wide.end());
+        }
+    }
     if (pk.geom.empty()) pk.geom.resize(8, 0.0f);
     if (pk.attrs.empty()) pk.attrs.resize(48, 0.0f);
 
@@ -574,6 +585,7 @@ void bind(const Packed &st, const EnvMap *env, const CameraFrame &cam, const oxp
     P.offs2[0] = has_env ? env_base + ew.mass : 0;
     P.offs2[1] = has_env ? env_base + ew.rad : 0;
     P.offs2[2] = st.tri_base;
+    P.offs2[3] = st.wide_base;
     P.fparams[0] = std::isfinite(o.ambient) ? std::max(o.ambient, 0.0f) : 0.0f;
     P.fparams[1] = o.clamp;
     P.fparams[2] = p_env;

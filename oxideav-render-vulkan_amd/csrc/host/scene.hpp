@@ -78,6 +78,8 @@ struct Packed {
     uint32_t n_punctual = 0, n_emissive = 0;
     uint32_t off_sobol = 0, off_sheen = 0, off_mat = 0, off_plights = 0, off_elights = 0;
     uint32_t bvh_depth = 0;
+    uint32_t wide_base = 0;   // first float4 of the 4-wide nodes in geom (0: none)
+    uint32_t wide_depth = 0;
 };
 
 struct EnvWords {
@@ -88,6 +90,14 @@ struct EnvWords {
 // Build the BVH (binned SAH, 32-byte nodes) over the soup: node words (8
 // per node) and the leaf-ordered triangle ids. Returns the tree depth.
 uint32_t build_bvh(const Soup &soup, std::vector<uint32_t> &node_words, std::vector<uint32_t> &order);
+
+// Traversal-stack entries of the 4-wide walk (the kernels' capacity).
+constexpr uint32_t kWideStack = 96;
+// Collapse the binary tree into 4-wide nodes (32 words each, see bvh.cpp).
+// Returns the wide depth, or 0 (and no nodes) when the tree does not fit
+// the wide walk's limits (leaves of at most 30 slots, 2^27 slots or nodes,
+// 3 * depth <= kWideStack); the kernels then walk the binary tree.
+uint32_t build_wide(const std::vector<uint32_t> &node_words, std::vector<float> &wide);
 
 // Pack everything but the environment (GpuPathTracer::upload).
 Packed pack_scene(const HostScene &s);

@@ -128,7 +128,7 @@ class COptions(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block", u32), ("waves_per_cu", u32), ("max_paths", u32), ("kernel", u32), ("wf_variant", u32),
-                ("stack_lds", u32), ("reserved", u32 * 2)]
+                ("stack_lds", u32), ("bvh", u32), ("reserved", u32)]
 
 
 STACK_PRIVATE = 0xFFFF  # oxpt_tuning.stack_lds: whole traversal stack in private memory
@@ -576,8 +576,8 @@ class GpuPathTracer:
         lib.oxpt_ctx_timing(self.ctx, 1 if reset else 0, C.byref(n), C.byref(ms))
         return n.value, ms.value
 
-    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds)
+    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh)
         rc = lib.oxpt_ctx_set_tuning(self.ctx, C.byref(t))
         if rc:
             _raise(rc, lib.oxpt_last_error(self.ctx).decode())
@@ -665,8 +665,8 @@ class Device:
         lib.oxpt_ctx_timing(self.h, 1 if reset else 0, C.byref(n), C.byref(ms))
         return n.value, ms.value
 
-    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds)
+    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh)
         self._check(lib.oxpt_ctx_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def get_tuning(self) -> Tuning:

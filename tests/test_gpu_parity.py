@@ -92,6 +92,37 @@ def test_closest_hit_matches_oracle(gpu):
     assert (ws != 0xFFFFFFFF).sum() > n // 4
 
 
+def test_wide_and_binary_walks_agree(gpu):
+    """The 4-wide walk reaches the same leaves through the same boxes as the
+    binary walk: same closest hits (slot and t bits) on random rays, and the
+    same image."""
+    scene = small_terrain()
+    opts = oxpt.RenderOptions(width=96, height=64, samples_per_pixel=4, max_bounces=8)
+    gpu.set_environment(oxpt.EnvironmentMap.sky(256, 128, 5))
+    gpu.invalidate_scene()
+    gpu.sync(scene, opts)
+    gpu.refine(1)
+    assert gpu.params().offs2[3] > 0, "scene carries no 4-wide tree"
+    rng = np.random.default_rng(5)
+    n = 1 << 16
+    o = np.stack([rng.uniform(-6, 6, n), rng.uniform(0.5, 8, n), rng.uniform(-3, 3, n)], axis=1).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    hits = {}
+    imgs = {}
+    for bvh in (0, 2):
+        gpu.set_tuning(bvh=bvh)
+        hits[bvh] = gpu.debug_closest_hit(o, d)
+        gpu.reset()
+        gpu.refine(4)
+        imgs[bvh] = gpu.hdr().view(np.uint32).copy()
+    gpu.set_tuning(bvh=0)
+    gpu.set_environment(None)
+    assert (hits[0][:, 0] != 0xFFFFFFFF).sum() > n // 8
+    assert np.array_equal(hits[0], hits[2]), f"{(hits[0] != hits[2]).any(axis=1).sum()} rays differ"
+    assert np.array_equal(imgs[0], imgs[2])
+
+
 def test_room_full_frame_psnr(gpu):
     """Plain small workload: the room at 256x256, 16 spp, depth 8 — full frame vs the oracle."""
     scene = oxpt.Scene.room(0)
@@ -164,6 +195,8 @@ def test_tuning_rejects_bad_stack_split(gpu):
         gpu.set_tuning(stack_lds=65)
     with pytest.raises(oxpt.Error):
         gpu.set_tuning(kernel=9)
+    with pytest.raises(oxpt.Error):
+        gpu.set_tuning(bvh=3)
     gpu.set_tuning(kernel=0)
 
 

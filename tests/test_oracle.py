@@ -191,10 +191,64 @@ def test_bvh_leaf_slots_cover_every_triangle_once():
             stack.append(ch)
     assert reached == int(leaves.sum())
     # Global ids in the triangle words are a permutation.
-    tris = pk.geom[8 * n_nodes:].reshape(-1, 12)
+    tris = pk.geom[8 * n_nodes: 8 * n_nodes + 12 * scene.triangle_count()].reshape(-1, 12)
     gids = tris[:, 3].view(np.uint32)
     assert np.array_equal(np.sort(gids), np.arange(scene.triangle_count(), dtype=np.uint32))
     assert pk.bvh_depth < 64
+
+
+@pytest.mark.parametrize("gen", ["terrain", "room", "coincident"])
+def test_wide_bvh_covers_every_slot_with_conservative_boxes(gen):
+    """The 4-wide nodes the kernels walk: every leaf slot reached exactly once,
+    every child box holds every triangle below it, child boxes taken from the
+    binary tree unchanged, the stack bound 3 * depth <= 96 holds."""
+    if gen == "terrain":
+        scene = oxpt.Scene.terrain(seed=2, gx=48, gz=32, n_spheres=10, subdiv=2, n_emissive=8)
+    elif gen == "room":
+        scene = oxpt.Scene.room(0)
+    else:
+        # 200 coincident triangles: SAH finds no split, the object-median
+        # fallback still bounds every leaf.
+        scene = oxpt.Scene()
+        m = scene.add_material(oxpt.material(base_color=(0.5, 0.5, 0.5, 1.0)))
+        scene.add_triangles(np.tile(np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32), (200, 1)), m)
+    pk = oxpt.Packed(scene, oxpt.RenderOptions(width=8, height=8))
+    P = pk.params
+    ntri = scene.triangle_count()
+    base = P.offs2[3]
+    assert base > 0 and base % 8 == 0
+    tri = pk.geom[8 * P.env[3]: 8 * P.env[3] + 12 * ntri].reshape(-1, 12)
+    pos = tri.reshape(-1, 3, 4)[:, :, :3]
+    wide = pk.geom[4 * base:].reshape(-1, 32)
+    bin_boxes = {tuple(np.concatenate([n[:3], n[4:7]]).view(np.uint32)) for n in pk.geom[: 8 * P.env[3]].reshape(-1, 8)}
+    covered = np.zeros(ntri, np.int32)
+    stack, max_depth = [(0, 1)], 0
+    while stack:
+        node, depth = stack.pop()
+        max_depth = max(max_depth, depth)
+        w = wide[node]
+        refs, cnts = w[24:28].view(np.uint32), w[28:32].view(np.uint32)
+        for i in range(4):
+            if cnts[i] == 0xFFFFFFFF:
+                continue
+            lo = np.array([w[i], w[4 + i], w[8 + i]], np.float32)
+            hi = np.array([w[12 + i], w[16 + i], w[20 + i]], np.float32)
+            if cnts[i] == 0:
+                cw = wide[refs[i]]
+                for j in range(4):
+                    if cw[28:32].view(np.uint32)[j] != 0xFFFFFFFF:
+                        clo = np.array([cw[j], cw[4 + j], cw[8 + j]])
+                        chi = np.array([cw[12 + j], cw[16 + j], cw[20 + j]])
+                        assert np.all(clo >= lo) and np.all(chi <= hi)
+                stack.append((refs[i], depth + 1))
+            else:
+                assert 1 <= cnts[i] <= 30
+                sl = slice(int(refs[i]), int(refs[i] + cnts[i]))
+                covered[sl] += 1
+                assert np.all(pos[sl] >= lo) and np.all(pos[sl] <= hi)
+            assert tuple(np.concatenate([lo, hi]).view(np.uint32)) in bin_boxes
+    assert (covered == 1).all()
+    assert 3 * max_depth <= 96
 
 
 # ---------------------------------------------------------------------------
@@ -256,7 +310,7 @@ def test_emissive_cdf_invariants():
     assert np.all(np.diff(cdf) >= 0) and cdf[-1] == 1.0
     assert np.allclose(np.diff(np.concatenate([[0.0], cdf])), pmf, atol=1e-7)
     # The slot of each record holds its global triangle id.
-    tris = pk.geom[8 * P.env[3]:].reshape(-1, 12)
+    tris = pk.geom[8 * P.env[3]: 8 * P.env[3] + 12 * scene.triangle_count()].reshape(-1, 12)
     for g, slot in rec[:, 2:4]:
         assert tris[slot, 3].view(np.uint32) == g
         assert tris[slot, 7].view(np.uint32) < n
