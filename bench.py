@@ -139,12 +139,86 @@ def np_equal(a, b):
     return a.shape == b.shape and bool((a.view(np.uint32) == b.view(np.uint32)).all())
 
 
+def bench_serving(args):
+    """Serving: Poisson render arrivals drawn from the uneven-batch
+    distribution plus refine(4) on a quarter of the rendered scenes, replayed
+    in real time against the continuous-batching server."""
+    import oxpt
+    from oxpt import workloads
+
+    t = time.perf_counter()
+    pool = workloads.uneven_batch(n=64, seed=3, threads=min(16, os.cpu_count() or 1), pack=False)
+    srv = oxpt.Server(0, hbm_budget=args.budget_gb << 30, spp_per_tick=args.spp_tick)
+    ids = [srv.add_scene(j.scene, j.env) for j in pool]
+    log(f"64 pool scenes generated + packed in {time.perf_counter() - t:.1f} s")
+    reqs = workloads.serving_stream(duration=args.duration, rate=args.rate, pool=len(pool))
+    n_render = sum(r.kind == "render" for r in reqs)
+    scene_of = {r.seq: r.job for r in reqs if r.kind == "render"}
+    offered = sum(pool[r.job].paths if r.kind == "render" else
+                  r.spp * pool[scene_of[r.job]].opts.width * pool[scene_of[r.job]].opts.height for r in reqs)
+    log(f"{len(reqs)} requests ({n_render} renders) over {args.duration:.0f} s at {args.rate} req/s, "
+        f"{offered / 1e9:.2f} G paths offered")
+    render_of = {}
+    handles = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        delay = t0 + r.arrival - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        if r.kind == "render":
+            h = srv.submit_render(ids[r.job], pool[r.job].opts)
+            render_of[r.seq] = h
+        else:
+            h = srv.submit_refine(render_of[r.job], r.spp)
+        handles.append(h)
+    log(f"all {len(handles)} requests submitted after {time.perf_counter() - t0:.1f} s")
+    lat, paths, failed = [], 0, 0
+    for i, h in enumerate(handles):
+        info = srv.wait(h, timeout=args.drain_timeout)
+        if not info.ok:
+            failed += 1
+            continue
+        lat.append(info.latency_s)
+        paths += info.paths
+        if (i + 1) % 100 == 0:
+            log(f"{i + 1}/{len(handles)} requests done")
+    t1 = time.perf_counter()
+    st = srv.stats()
+    lat_ms = sorted(x * 1e3 for x in lat)
+
+    def pct(q):
+        return round(lat_ms[min(len(lat_ms) - 1, int(round(q / 100.0 * (len(lat_ms) - 1))))], 3) if lat_ms else None
+
+    busy = st.busy_s
+    out = {
+        "metric": "served Mpaths/s; p50/p99 request latency (arrival to result ready)",
+        "value": round(paths / (t1 - t0) / 1e6, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": len(handles),
+        "warmup": 0, "ms_per_step": round((t1 - t0) * 1e3 / max(len(handles), 1), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene generators)",
+        "config": {"workload": f"serving: Poisson {args.rate} req/s for {args.duration:.0f} s from the uneven-batch "
+                               f"distribution (64-scene pool) + refine(4) on 25% of renders; HBM budget "
+                               f"{args.budget_gb} GB; {args.spp_tick} spp per tick",
+                   "requests": len(handles), "failed": failed, "offered_paths": int(offered),
+                   "offered_mpaths_s": round(offered / args.duration / 1e6, 3)},
+        "latency_ms": {"p50": pct(50), "p99": pct(99)},
+        "busy_mpaths_s": round(st.paths / busy / 1e6, 3) if busy > 0 else None,
+        "server": {"ticks": st.ticks, "uploads": st.uploads, "evictions": st.evictions, "busy_s": round(busy, 3),
+                   "wall_s": round(t1 - t0, 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3", "c4"])
+    ap.add_argument("--rate", type=float, default=50.0, help="c4: Poisson arrival rate (req/s)")
+    ap.add_argument("--duration", type=float, default=60.0, help="c4: arrival window (s)")
+    ap.add_argument("--budget-gb", type=int, default=200, help="c4: HBM admission budget")
+    ap.add_argument("--spp-tick", type=int, default=16, help="c4: samples per request per tick")
+    ap.add_argument("--drain-timeout", type=float, default=600.0, help="c4: wait per request (s)")
     ap.add_argument("--cpu-tiles", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg (PSNR, baseline, bytes/path)")
     ap.add_argument("--kernel", type=int, default=0, help="0 persistent megakernel, 1 plain per-pixel kernel")
@@ -175,6 +249,8 @@ def main():
 
     if args.workload == "c3":
         return bench_uneven_batch(args)
+    if args.workload == "c4":
+        return bench_serving(args)
     scene, env, opts, desc = workload(args.workload)
     pt = oxpt.GpuPathTracer(0)
     log("device:", pt.adapter_summary())

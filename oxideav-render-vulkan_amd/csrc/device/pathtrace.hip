@@ -483,16 +483,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) tr
 }
 
 // Batched form: many (scene, camera, sample run) jobs in one persistent
-// launch. A wave works on one job at a time (so the job's Params and scene
-// pointers stay in scalar registers) and its idle lanes claim that job's
-// pixels with the ballot + one-atomic scheme above; when the job runs dry
-// and the wave's lanes have finished their pixels, the wave moves to the
-// next job that still has work. Jobs are ordered largest first by the
-// host, so the small ones fill the tail. Per-pixel sample order is kept,
-// so every job's image equals its image alone.
+// launch. The work item is one path — (job, 8x8 tile, sample, pixel), so a
+// wave's 64 lanes take one tile at one sample index — and a lane claims
+// its next item as soon as its path ends. A wave works on one job at a time
+// (so the job's Params and scene pointers stay in scalar registers), and
+// when the job runs dry it moves on after at most one path per lane, not a
+// whole run of samples per pixel. Each path writes (rgb, covered) to the
+// launch's sample buffer; batch_reduce then adds every pixel's samples into
+// its accumulator in sample order, so every job's image equals its image
+// alone. Jobs are ordered largest first by the host.
 template <int MinWaves>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves)))
-trace_batch(BatchArgs B, uint32_t lds_depth) {
+trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
     extern __shared__ uint64_t lds_dyn[];
@@ -500,75 +502,90 @@ trace_batch(BatchArgs B, uint32_t lds_depth) {
     // Every job carries the same estimator constants (checked on the host).
     const uint32_t n_jobs = min(*B.n_jobs, B.max_jobs);
     if (n_jobs == 0) return;
-    stage_tables(B.jobs[0].tables, B.jobs[0].P, lds_sobol, lds_sheen);
+    stage_tables(jobs[0].tables, jobs[0].P, lds_sobol, lds_sheen);
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
-    uint32_t j = 0;
-    bool job_live = true;  // wave-uniform: job j may still have unclaimed pixels
-    bool has_pixel = false, need_init = false;
-    uint32_t px = 0, py = 0, done = 0;
-    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool has_path = false;
+    uint32_t item = 0;
     PathState st;
+    uint32_t j = 0;
     for (;;) {
         j = __builtin_amdgcn_readfirstlane(j);
-        const Job &J = B.jobs[j];
+        const Job &J = jobs[j];
+        const Scene S = make_scene(J.P, J.geom, J.attrs, J.tables, J.atlas, J.atlas_side, lds_sheen);
         const uint32_t w = J.P.dims[0], h = J.P.dims[1];
-        bool want = !has_pixel && job_live;
-        uint64_t m = __ballot(want);
-        if (m) {
-            uint32_t leader = __ffsll((unsigned long long)m) - 1u;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(B.counters + j, (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            bool dry = false;
-            if (want) {
-                uint32_t item = base + (uint32_t)__popcll(m & lt_mask);
-                if (item >= J.n_items) {
-                    dry = true;
-                } else {
-                    item_pixel(J.tiles_x, item, px, py);
-                    if (px < w && py < h) {
-                        has_pixel = true;
-                        done = 0;
-                        need_init = true;
-                        acc = J.accum[(size_t)py * w + px];
+        const uint32_t per_tile = 64u * J.count, n_paths = J.n_items * J.count;
+        bool job_live = true;  // wave-uniform: job j may still have unclaimed paths
+        for (;;) {
+            bool want = !has_path && job_live;
+            uint64_t m = __ballot(want);
+            if (m) {
+                uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(B.counters + j, (uint32_t)__popcll(m));
+                base = __shfl(base, leader);
+                bool dry = false;
+                if (want) {
+                    const uint32_t it = base + (uint32_t)__popcll(m & lt_mask);
+                    if (it >= n_paths) {
+                        dry = true;
+                    } else {
+                        const uint32_t tile = it / per_tile, rem = it - tile * per_tile;
+                        const uint32_t s = rem >> 6, in = rem & 63u;
+                        const uint32_t px = (tile % J.tiles_x) * 8u + (in & 7u);
+                        const uint32_t py = (tile / J.tiles_x) * 8u + (in >> 3);
+                        // Items outside the image (ragged edge tiles) have
+                        // no path; the reduce never reads them.
+                        if (px < w && py < h) {
+                            has_path = true;
+                            item = it;
+                            init_path(J.P, lds_sobol, px, py, J.first + s, st);
+                        }
                     }
                 }
+                if (__ballot(dry)) job_live = false;
             }
-            if (__ballot(dry)) job_live = false;
-        }
-        if (__ballot(has_pixel) == 0) {
-            if (job_live) continue;  // ragged edge items only: claim again
-            // Next job with unclaimed work.
-            uint32_t nj = j + 1;
-            while (nj < n_jobs && __hip_atomic_load(B.counters + nj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
-                                        B.jobs[nj].n_items)
-                nj++;
-            if (nj >= n_jobs) break;
-            j = nj;
-            job_live = true;
-            continue;
-        }
-        if (!has_pixel) continue;
-        const Scene S = make_scene(J.P, J.geom, J.attrs, J.tables, J.atlas, J.atlas_side, lds_sheen);
-        if (need_init) {
-            init_path(J.P, lds_sobol, px, py, J.first + done, st);
-            need_init = false;
-        }
-        bool covered;
-        if (step_path(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) {
-            if (covered) {
-                F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
-                acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
+            if (__ballot(has_path) == 0) {
+                if (job_live) continue;  // ragged edge items only: claim again
+                break;                   // job j is drained for this wave
             }
-            done += 1;
-            if (done == J.count) {
-                J.accum[(size_t)py * w + px] = acc;
-                has_pixel = false;
-            } else {
-                need_init = true;
+            if (!has_path) continue;
+            bool covered;
+            if (step_path(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) {
+                F3 r = covered && finite3(st.radiance) ? st.radiance : f3(0.0f);
+                B.samples[J.sample_base + item] = make_float4(r.x, r.y, r.z, covered ? 1.0f : 0.0f);
+                has_path = false;
             }
         }
+        // Next job with unclaimed work.
+        uint32_t nj = j + 1;
+        while (nj < n_jobs &&
+               __hip_atomic_load(B.counters + nj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= jobs[nj].n_items * jobs[nj].count)
+            nj++;
+        if (nj >= n_jobs) break;
+        j = nj;
+    }
+}
+
+// Per pixel of every job (blockIdx.y = job): the launch's samples, in sample
+// order, into the accumulator. The grid is fixed (grid-stride over pixels,
+// jobs past the device-side count exit), so the step stays capturable.
+__global__ void __launch_bounds__(256) batch_reduce(BatchArgs B, const Job *__restrict__ jobs) {
+    const uint32_t jb = blockIdx.y;
+    if (jb >= min(*B.n_jobs, B.max_jobs)) return;
+    const Job &J = jobs[jb];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < J.n_items; i += gridDim.x * blockDim.x) {
+        const uint32_t tile = i >> 6, in = i & 63u;
+        const uint32_t px = (tile % J.tiles_x) * 8u + (in & 7u), py = (tile / J.tiles_x) * 8u + (in >> 3);
+        if (px >= J.P.dims[0] || py >= J.P.dims[1]) continue;
+        const size_t at = (size_t)py * J.P.dims[0] + px;
+        float4 acc = J.accum[at];
+        const float4 *smp = B.samples + J.sample_base + (size_t)tile * 64u * J.count + in;
+        for (uint32_t s = 0; s < J.count; s++) {
+            const float4 r = smp[(size_t)s * 64u];
+            if (r.w > 0.0f) acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
+        }
+        J.accum[at] = acc;
     }
 }
 
@@ -727,13 +744,16 @@ hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t b
     if (e != hipSuccess) return e;
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
     switch (kernel) {
-        case 3: hipLaunchKernelGGL(trace_batch<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        case 4: hipLaunchKernelGGL(trace_batch<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        case 5: hipLaunchKernelGGL(trace_batch<6>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        case 6: hipLaunchKernelGGL(trace_batch<8>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        case 8: hipLaunchKernelGGL(trace_batch<2>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        default: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 3: hipLaunchKernelGGL(trace_batch<4>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        case 4: hipLaunchKernelGGL(trace_batch<5>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        case 5: hipLaunchKernelGGL(trace_batch<6>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        case 6: hipLaunchKernelGGL(trace_batch<8>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        case 8: hipLaunchKernelGGL(trace_batch<2>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        default: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
     }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_reduce, dim3(64, args.max_jobs), dim3(256), 0, s, args, args.jobs);
     return hipGetLastError();
 }
 

@@ -575,6 +575,79 @@ __device__ __forceinline__ bool leaf_hits(const Scene &S, const Ray &r, uint32_t
 constexpr uint32_t WIDE_STACK = 96;
 constexpr uint32_t WIDE_EMPTY = 0xffffffffu;
 
+// One step of the 4-wide walk from the node or leaf (first, count): a leaf
+// is tested, an inner node's hit children are ordered and the nearest
+// entered; when nothing is entered the stack is popped. Returns false when
+// the walk is over (stack empty, or an any-hit query answered).
+__device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t_max, Hit &best, uint32_t &first,
+                                          uint32_t &count, uint32_t &sp, uint32_t path_seed, uint32_t ray_id,
+                                          bool primary, bool any, uint32_t skip, lds_u64 *lds, uint64_t *priv,
+                                          uint32_t in_lds, uint32_t priv_off) {
+    if (count > 0) {
+        if (leaf_hits(S, r, first, count, t_max, best, path_seed, ray_id, primary, any, skip)) return false;
+    } else {
+        const gf4_ptr nd = S.geom + S.wide_base + 8u * first;
+        const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
+        const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
+        const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
+        // Sort keys: (|t| bits << 32 | child) for hit children, all ones for misses.
+        uint64_t k[4];
+        {
+            const float t0 = cn.x != WIDE_EMPTY ? slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f),
+                                                       make_float4(hx.x, hy.x, hz.x, 0.0f), t_max) : -1.0f;
+            const float t1 = cn.y != WIDE_EMPTY ? slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f),
+                                                       make_float4(hx.y, hy.y, hz.y, 0.0f), t_max) : -1.0f;
+            const float t2 = cn.z != WIDE_EMPTY ? slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f),
+                                                       make_float4(hx.z, hy.z, hz.z, 0.0f), t_max) : -1.0f;
+            const float t3 = cn.w != WIDE_EMPTY ? slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f),
+                                                       make_float4(hx.w, hy.w, hz.w, 0.0f), t_max) : -1.0f;
+            k[0] = t0 >= 0.0f ? ((uint64_t)(fbits(t0) & 0x7fffffffu) << 32) | 0u : ~0ull;
+            k[1] = t1 >= 0.0f ? ((uint64_t)(fbits(t1) & 0x7fffffffu) << 32) | 1u : ~0ull;
+            k[2] = t2 >= 0.0f ? ((uint64_t)(fbits(t2) & 0x7fffffffu) << 32) | 2u : ~0ull;
+            k[3] = t3 >= 0.0f ? ((uint64_t)(fbits(t3) & 0x7fffffffu) << 32) | 3u : ~0ull;
+        }
+#define OXD_CSWAP(a, b)                                 \
+    {                                                   \
+        const uint64_t lo_ = k[a] < k[b] ? k[a] : k[b]; \
+        const uint64_t hi_ = k[a] < k[b] ? k[b] : k[a]; \
+        k[a] = lo_;                                     \
+        k[b] = hi_;                                     \
+    }
+        OXD_CSWAP(0, 1) OXD_CSWAP(2, 3) OXD_CSWAP(0, 2) OXD_CSWAP(1, 3) OXD_CSWAP(1, 2)
+#undef OXD_CSWAP
+        if (k[0] != ~0ull) {
+            // Push the farther hit children, farthest first.
+#pragma unroll
+            for (int j = 3; j >= 1; j--) {
+                if (k[j] == ~0ull) continue;
+                const uint32_t c = (uint32_t)k[j] & 3u;
+                const uint32_t cref = c == 0 ? rf.x : c == 1 ? rf.y : c == 2 ? rf.z : rf.w;
+                const uint32_t ccnt = c == 0 ? cn.x : c == 1 ? cn.y : c == 2 ? cn.z : cn.w;
+                if (sp < WIDE_STACK) {
+                    const uint64_t e = (k[j] & 0xffffffff00000000ull) | (cref << 5 | ccnt);
+                    if (sp < in_lds) lds[sp * 64u] = e;
+                    else priv[sp - priv_off] = e;
+                    sp++;
+                }
+            }
+            const uint32_t c = (uint32_t)k[0] & 3u;
+            first = c == 0 ? rf.x : c == 1 ? rf.y : c == 2 ? rf.z : rf.w;
+            count = c == 0 ? cn.x : c == 1 ? cn.y : c == 2 ? cn.z : cn.w;
+            return true;
+        }
+    }
+    while (sp != 0) {
+        sp--;
+        const uint64_t e = sp < in_lds ? lds[sp * 64u] : priv[sp - priv_off];
+        if (bitsf((uint32_t)(e >> 32)) <= t_max) {
+            first = (uint32_t)e >> 5;
+            count = (uint32_t)e & 31u;
+            return true;
+        }
+    }
+    return false;
+}
+
 template <int LdsDepth>
 __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                          uint32_t ray_id, bool primary, bool any, uint32_t skip, lds_u64 *lds_stack,
@@ -592,74 +665,8 @@ __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_m
     uint64_t priv[LdsDepth > 0 ? WIDE_STACK - LdsDepth : WIDE_STACK];
     constexpr uint32_t priv_off = LdsDepth > 0 ? (uint32_t)LdsDepth : 0u;
     uint32_t sp = 0, first = 0, count = 0;  // wide node 0, inner
-    for (;;) {
-        bool descended = false;
-        if (count > 0) {
-            if (leaf_hits(S, r, first, count, t_max, best, path_seed, ray_id, primary, any, skip)) break;
-        } else {
-            const gf4_ptr nd = S.geom + S.wide_base + 8u * first;
-            const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
-            const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
-            const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
-            // Sort keys: (|t| bits << 32 | child) for hit children, all ones for misses.
-            uint64_t k[4];
-            {
-                const float t0 = cn.x != WIDE_EMPTY ? slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f),
-                                                           make_float4(hx.x, hy.x, hz.x, 0.0f), t_max) : -1.0f;
-                const float t1 = cn.y != WIDE_EMPTY ? slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f),
-                                                           make_float4(hx.y, hy.y, hz.y, 0.0f), t_max) : -1.0f;
-                const float t2 = cn.z != WIDE_EMPTY ? slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f),
-                                                           make_float4(hx.z, hy.z, hz.z, 0.0f), t_max) : -1.0f;
-                const float t3 = cn.w != WIDE_EMPTY ? slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f),
-                                                           make_float4(hx.w, hy.w, hz.w, 0.0f), t_max) : -1.0f;
-                k[0] = t0 >= 0.0f ? ((uint64_t)(fbits(t0) & 0x7fffffffu) << 32) | 0u : ~0ull;
-                k[1] = t1 >= 0.0f ? ((uint64_t)(fbits(t1) & 0x7fffffffu) << 32) | 1u : ~0ull;
-                k[2] = t2 >= 0.0f ? ((uint64_t)(fbits(t2) & 0x7fffffffu) << 32) | 2u : ~0ull;
-                k[3] = t3 >= 0.0f ? ((uint64_t)(fbits(t3) & 0x7fffffffu) << 32) | 3u : ~0ull;
-            }
-#define OXD_CSWAP(a, b)                            \
-    {                                              \
-        const uint64_t lo_ = k[a] < k[b] ? k[a] : k[b]; \
-        const uint64_t hi_ = k[a] < k[b] ? k[b] : k[a]; \
-        k[a] = lo_;                                \
-        k[b] = hi_;                                \
-    }
-            OXD_CSWAP(0, 1) OXD_CSWAP(2, 3) OXD_CSWAP(0, 2) OXD_CSWAP(1, 3) OXD_CSWAP(1, 2)
-#undef OXD_CSWAP
-            if (k[0] != ~0ull) {
-                // Push the farther hit children, farthest first.
-#pragma unroll
-                for (int j = 3; j >= 1; j--) {
-                    if (k[j] == ~0ull) continue;
-                    const uint32_t c = (uint32_t)k[j] & 3u;
-                    const uint32_t cref = c == 0 ? rf.x : c == 1 ? rf.y : c == 2 ? rf.z : rf.w;
-                    const uint32_t ccnt = c == 0 ? cn.x : c == 1 ? cn.y : c == 2 ? cn.z : cn.w;
-                    if (sp < WIDE_STACK) {
-                        const uint64_t e = (k[j] & 0xffffffff00000000ull) | (cref << 5 | ccnt);
-                        if (sp < in_lds) lds[sp * 64u] = e;
-                        else priv[sp - priv_off] = e;
-                        sp++;
-                    }
-                }
-                const uint32_t c = (uint32_t)k[0] & 3u;
-                first = c == 0 ? rf.x : c == 1 ? rf.y : c == 2 ? rf.z : rf.w;
-                count = c == 0 ? cn.x : c == 1 ? cn.y : c == 2 ? cn.z : cn.w;
-                descended = true;
-            }
-        }
-        if (descended) continue;
-        bool found = false;
-        while (sp != 0) {
-            sp--;
-            const uint64_t e = sp < in_lds ? lds[sp * 64u] : priv[sp - priv_off];
-            if (bitsf((uint32_t)(e >> 32)) <= t_max) {
-                first = (uint32_t)e >> 5;
-                count = (uint32_t)e & 31u;
-                found = true;
-                break;
-            }
-        }
-        if (!found) break;
+    while (wide_step(S, r, t_max, best, first, count, sp, path_seed, ray_id, primary, any, skip, lds, priv, in_lds,
+                     priv_off)) {
     }
     return best;
 }

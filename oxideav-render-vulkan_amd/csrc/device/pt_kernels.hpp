@@ -24,6 +24,7 @@ struct TraceArgs {
 
 // One job of a batched launch: a resident scene, its Params, an accumulator
 // and a sample run. 256 bytes, read with scalar loads by the wave that owns it.
+// sample_base: the job's first slot in the launch's per-path sample buffer.
 struct alignas(16) Job {
     oxpt_params P;  // 176 B
     const float4 *geom;
@@ -32,7 +33,8 @@ struct alignas(16) Job {
     const uint16_t *atlas;
     float4 *accum;
     uint32_t atlas_side, first, count, tiles_x, n_items;
-    uint32_t pad[5];
+    uint32_t sample_base;
+    uint32_t pad[4];
 };
 static_assert(sizeof(Job) == 256, "Job is 256 bytes");
 
@@ -44,6 +46,7 @@ struct BatchArgs {
     const uint32_t *n_jobs;
     uint32_t *counters;  // one work counter per job
     uint32_t max_jobs;   // capacity of jobs / counters
+    float4 *samples;     // one (rgb, covered) per traced path of the launch
 };
 
 // lds_depth: traversal-stack entries per lane held in LDS (dynamic shared

@@ -71,6 +71,8 @@ struct oxpt_ctx {
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;  // captured batched step (reset + trace)
     uint32_t graph_block = 0, graph_grid = 0, graph_kernel = 0, graph_lds = 0;
+    float4 *d_samples = nullptr;  // batched launch: one (rgb, covered) per path of a round
+    size_t samples_bytes = 0;
     uint64_t graph_launches = 0;
     void *d_out = nullptr;
     size_t d_out_bytes = 0;
@@ -244,7 +246,10 @@ int wf_run(oxpt_ctx *c, const std::vector<oxd::Job> &jobs) {
     for (auto &J : jobs) items_max += (uint64_t)J.n_items * std::min(spd, J.count);
     if (items_max == 0) return OXPT_OK;
     if (items_max >= 0xffffffffull) return fail(c, OXPT_E_ARG, "wavefront: pass too large");
-    uint32_t want_slots = (uint32_t)std::min<uint64_t>(1ull << 21, (items_max + 255) / 256 * 256);
+    // Slots x shadow rays per slot stays below 2^31 (32-bit ray ids).
+    const uint64_t slot_cap = std::min<uint64_t>(1ull << 21, ((1ull << 31) / max_shadow) / 256 * 256);
+    if (slot_cap == 0) return fail(c, OXPT_E_ARG, "wavefront: too many punctual lights");
+    uint32_t want_slots = (uint32_t)std::min<uint64_t>(slot_cap, (items_max + 255) / 256 * 256);
     if (W.n_slots < want_slots || W.max_shadow < max_shadow) {
         uint32_t ns = std::max(want_slots, W.n_slots), ms = std::max(max_shadow, W.max_shadow);
         if (int rc = wf_grow(c, W.slots, W.slot_bytes, (size_t)ns * sizeof(oxd::PathSlot), "wavefront slots")) return rc;
@@ -396,6 +401,7 @@ void oxpt_ctx_destroy(oxpt_ctx *c) {
     }
     dfree(c, c->d_jobs, c->job_cap * sizeof(oxd::Job));
     dfree(c, c->d_counters, c->job_cap * 4 + 4);
+    dfree(c, c->d_samples, c->samples_bytes);
     dfree(c, c->d_out, c->d_out_bytes);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -698,7 +704,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         c->d_counters = (uint32_t *)p;
         c->job_cap = cap;
     }
-    if (c->tuning.kernel == 7 || (c->tuning.kernel == 0 && !c->use_graph)) {
+    if (c->tuning.kernel == 7) {
         std::vector<oxd::Job> jobs;
         for (size_t i = 0; i < n; i++) {
             const oxpt_batch_item &it = items[i];
@@ -743,6 +749,28 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
     const uint32_t lds = stack_lds_entries(c->tuning, block, true);
+    // One sample slot per traced path of a round (8x8 tiles incl. ragged
+    // edges x samples of the round), sized for the largest round.
+    uint64_t slots = 0;
+    for (size_t i = 0; i < n; i++) {
+        const uint64_t tiles = (uint64_t)((items[i].accum->w + 7u) / 8u) * ((items[i].accum->h + 7u) / 8u);
+        slots += 64u * tiles * std::min(spd, items[i].count);
+    }
+    if (slots >= (1ull << 32)) return fail(c, OXPT_E_ARG, "batch: round too large (lower max_paths)");
+    if (c->samples_bytes < slots * 16) {
+        (void)hipStreamSynchronize(c->stream);
+        if (c->graph) {
+            (void)hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+        }
+        dfree(c, c->d_samples, c->samples_bytes);
+        c->d_samples = nullptr;
+        c->samples_bytes = 0;
+        void *p = nullptr;
+        if (int rc = dalloc(c, slots * 16, &p, "batch sample buffer")) return rc;
+        c->d_samples = (float4 *)p;
+        c->samples_bytes = slots * 16;
+    }
     std::vector<oxd::Job> jobs;
     std::vector<double> work;
     for (uint32_t done = 0; done < max_count; done += spd) {
@@ -774,14 +802,19 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         for (size_t k = 0; k < order.size(); k++) order[k] = k;
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
         std::vector<oxd::Job> sorted(jobs.size());
-        for (size_t k = 0; k < order.size(); k++) sorted[k] = jobs[order[k]];
+        uint32_t sample_base = 0;
+        for (size_t k = 0; k < order.size(); k++) {
+            sorted[k] = jobs[order[k]];
+            sorted[k].sample_base = sample_base;
+            sample_base += sorted[k].n_items * sorted[k].count;
+        }
         uint32_t nj = (uint32_t)sorted.size();
         hipError_t e = hipMemcpyAsync(c->d_jobs, sorted.data(), sorted.size() * sizeof(oxd::Job), hipMemcpyHostToDevice,
                                       c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(c->d_counters + c->job_cap, &nj, 4, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch job upload", e);
-        oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap};
+        oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap, c->d_samples};
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);

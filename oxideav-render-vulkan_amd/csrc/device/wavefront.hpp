@@ -48,6 +48,7 @@ struct WfArgs {
 };
 
 hipError_t wf_launch_clear(const WfArgs &A, hipStream_t s);
+// variant: see oxpt_tuning.wf_variant.
 hipError_t wf_launch_iteration(const WfArgs &A, uint32_t variant, hipStream_t s);
 hipError_t wf_launch_reduce(const WfArgs &A, uint32_t job, uint32_t n_items, hipStream_t s);
 

@@ -677,6 +677,13 @@ class Device:
     def set_budget(self, nbytes: int) -> None:
         self._check(lib.oxpt_ctx_set_budget(self.h, nbytes), "set_budget")
 
+    def set_graph(self, enable: bool) -> None:
+        """Replay the batched step from a captured HIP graph."""
+        self._check(lib.oxpt_ctx_set_graph(self.h, 1 if enable else 0), "set_graph")
+
+    def graph_launches(self) -> int:
+        return lib.oxpt_ctx_graph_launches(self.h)
+
     def bytes_in_use(self) -> int:
         return lib.oxpt_ctx_bytes_in_use(self.h)
 

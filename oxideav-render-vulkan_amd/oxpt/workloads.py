@@ -69,36 +69,39 @@ def make_job(n_tris: int, side: int, spp: int, seed: int, pack: bool = True) -> 
     return Job(scene, env, opts, pk, scene.triangle_count())
 
 
-def uneven_batch(n: int = 64, seed: int = 3, threads: int = 16, scale: float = 1.0) -> List[Job]:
-    """`scale` < 1 shrinks triangle counts and sides (for tests)."""
+def uneven_batch(n: int = 64, seed: int = 3, threads: int = 16, scale: float = 1.0, pack: bool = True) -> List[Job]:
+    """`scale` < 1 shrinks triangle counts and sides (for tests); `pack`
+    False leaves the packing to the consumer (the server packs on add)."""
     rng = np.random.default_rng(seed)
     draws = [draw_uneven(rng) for _ in range(n)]
     if scale != 1.0:
         draws = [(max(200, int(t * scale)), max(16, int(s * math.sqrt(scale))), p) for t, s, p in draws]
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        return list(ex.map(lambda a: make_job(a[1][0], a[1][1], a[1][2], seed * 1000 + a[0]), enumerate(draws)))
+        return list(ex.map(lambda a: make_job(a[1][0], a[1][1], a[1][2], seed * 1000 + a[0], pack), enumerate(draws)))
 
 
 @dataclass
 class Request:
     arrival: float      # seconds from the start
     kind: str           # "render" or "refine"
-    job: int            # index into the scene pool (render) or a resident request id (refine)
+    job: int            # pool scene (render) or the sequence number of the render a refine continues
     spp: int
+    seq: int = 0        # sequence number of a render (renders only)
 
 
 def serving_stream(duration: float, rate: float, pool: int, refine_frac: float = 0.25, seed: int = 9):
     """Poisson arrivals at `rate` req/s for `duration` s: renders of a pool
     scene; after each render a refine(4) follows with probability
-    `refine_frac` (on a then-resident scene)."""
+    `refine_frac` (on the then-resident scene of that render)."""
     rng = np.random.default_rng(seed)
-    t, out = 0.0, []
+    t, out, n_render = 0.0, [], 0
     while True:
         t += rng.exponential(1.0 / rate)
         if t > duration:
             break
-        out.append(Request(t, "render", int(rng.integers(0, pool)), 0))
+        out.append(Request(t, "render", int(rng.integers(0, pool)), 0, n_render))
         if rng.random() < refine_frac:
-            out.append(Request(t + rng.exponential(1.0 / rate), "refine", len(out) - 1, 4))
+            out.append(Request(t + rng.exponential(1.0 / rate), "refine", n_render, 4))
+        n_render += 1
     out.sort(key=lambda r: r.arrival)
     return out

@@ -67,3 +67,25 @@ def test_batch_rejects_bad_items():
     oacc = other.accum(jobs[0].opts.width, jobs[0].opts.height)
     with pytest.raises(oxpt.Error):
         dev.refine_batch([(ds[0], oacc, ds[0].params, 0, 1)])
+
+
+def test_batch_forms_bitwise():
+    """Every batched form (occupancy builds, the wavefront engine, the binary
+    tree walk, graph replay) gives each job's image alone."""
+    dev, jobs, ds, acc = _setup(n=8)
+    alone = []
+    for j, s, a in zip(jobs, ds, acc):
+        dev.refine(s, a, s.params, 0, j.opts.samples_per_pixel)
+        alone.append(a.read())
+    items = [(s, a, s.params, 0, j.opts.samples_per_pixel) for j, s, a in zip(jobs, ds, acc)]
+    for kernel, bvh, graph in ((3, 0, False), (4, 0, False), (8, 0, False), (7, 0, False), (0, 2, False),
+                               (0, 0, True), (0, 0, True)):
+        for a in acc:
+            a.clear()
+        dev.set_tuning(kernel=kernel, bvh=bvh)
+        dev.set_graph(graph)
+        dev.refine_batch(items)
+        for i, a in enumerate(acc):
+            assert np.array_equal(a.read().view(np.uint32), alone[i].view(np.uint32)), (kernel, bvh, graph, i)
+    dev.set_graph(False)
+    dev.set_tuning(kernel=0)

@@ -20,13 +20,15 @@ def _scenes():
            oxpt.RenderOptions(width=72, height=40, samples_per_pixel=4, max_bounces=16))
 
 
-@pytest.mark.parametrize("case", range(5))
-def test_wavefront_equals_megakernel(gpu, case):
+# wf_variant: traversal / shade occupancy forms; bvh=2: binary tree walk.
+@pytest.mark.parametrize("case,variant,bvh", [(c, 0x00, 0) for c in range(5)] +
+                         [(1, 0x02, 0), (1, 0x03, 0), (1, 0x11, 0), (1, 0x00, 2), (3, 0x00, 2)])
+def test_wavefront_equals_megakernel(gpu, case, variant, bvh):
     name, scene, env, opts = list(_scenes())[case]
     gpu.set_environment(env)
     gpu.set_tuning(kernel=0)
     a = gpu.render_hdr(scene, opts)
-    gpu.set_tuning(kernel=7)
+    gpu.set_tuning(kernel=7, wf_variant=variant, bvh=bvh)
     b = gpu.render_hdr(scene, opts)
     # Progressive split through the wavefront too.
     gpu.reset()
@@ -49,10 +51,13 @@ def test_wavefront_batch_equals_alone():
         dev.refine(s, a, s.params, 0, j.opts.samples_per_pixel)
         alone.append(a.read())
         a.clear()
-    dev.set_tuning(kernel=7)
-    dev.refine_batch([(s, a, s.params, 0, j.opts.samples_per_pixel) for j, s, a in zip(jobs, ds, acc)])
-    for i, a in enumerate(acc):
-        assert np.array_equal(a.read().view(np.uint32), alone[i].view(np.uint32)), i
+    for variant in (0x00, 0x11):
+        for a in acc:
+            a.clear()
+        dev.set_tuning(kernel=7, wf_variant=variant)
+        dev.refine_batch([(s, a, s.params, 0, j.opts.samples_per_pixel) for j, s, a in zip(jobs, ds, acc)])
+        for i, a in enumerate(acc):
+            assert np.array_equal(a.read().view(np.uint32), alone[i].view(np.uint32)), (i, variant)
     # Passes: a small path cap splits the samples over several passes.
     for a in acc:
         a.clear()
