@@ -48,6 +48,13 @@ def workload(name):
         opts = oxpt.RenderOptions(width=1920, height=1080, samples_per_pixel=64, max_bounces=12, rr_start=3)
         desc = ("single large scene: fBm heightfield 1024x512 quads + 200 icospheres (subdiv 3) + 64 emissive "
                 "tris, MR materials, 1024x512 sky CDF; 1920x1080, 64 spp, depth 12")
+    elif name == "c2":
+        scene = oxpt.Scene.layered(seed=2, n_tris=4_000_000, n_textures=16, tex_size=2048, n_lights=16)
+        env = None
+        opts = oxpt.RenderOptions(width=1920, height=1080, samples_per_pixel=256, max_bounces=16, rr_start=3)
+        desc = ("layered/texture scene: 4 M tris baked from instanced meshes, materials 30% transmission+volume, "
+                "20% clearcoat, 20% sheen, 10% alpha BLEND, 20% MR; 16 value-noise RGBA 2048^2 textures with "
+                "mips in the f16 atlas; 16 punctual lights; 1920x1080, 256 spp, depth 16")
     elif name == "c0":
         scene = oxpt.Scene.room(0)
         env = None
@@ -56,6 +63,22 @@ def workload(name):
     else:
         raise SystemExit(f"unknown workload {name}")
     return scene, env, opts, desc
+
+
+def pmc_traffic(args, paths_per_launch):
+    """HBM bytes per launch of the default kernel on the default workload,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE readings of that
+    launch, each scaled by its reading on a plain copy of known size
+    (profiles/counter_calibration_copy_1GiB.md). None for other configs."""
+    path = os.path.join(ROOT, "profiles", "trace_persistent_c1_pmc.json")
+    if args.workload != "c1" or args.kernel != 0 or args.spp or args.block or args.waves_per_cu:
+        return None
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    per_path = (d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"]) * 1024.0 / d["paths_per_dispatch"]
+    return int(round(per_path * paths_per_launch))
 
 
 def tiles(w, h, n, size=64, seed=1234):
@@ -213,7 +236,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--rate", type=float, default=50.0, help="c4: Poisson arrival rate (req/s)")
     ap.add_argument("--duration", type=float, default=60.0, help="c4: arrival window (s)")
     ap.add_argument("--budget-gb", type=int, default=200, help="c4: HBM admission budget")
@@ -334,7 +357,8 @@ def main():
         achieved = bytes_per_path * per_launch_paths / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": pmc_traffic(args, per_launch_paths),
             "kernel": "trace_persistent" if args.kernel == 0 else "trace_tiles",
             "bytes_per_path": round(bytes_per_path, 1), "paths_per_launch": int(per_launch_paths),
             "avg_launch_ms": round(avg_launch_s * 1e3, 3),

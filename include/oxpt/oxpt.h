@@ -153,6 +153,13 @@ typedef struct oxpt_tuning {
 #define OXPT_STACK_PRIVATE 0xFFFFu
 int oxpt_ctx_set_tuning(oxpt_ctx *ctx, const oxpt_tuning *t);
 int oxpt_ctx_get_tuning(const oxpt_ctx *ctx, oxpt_tuning *t);
+/* Autotuner (no counterpart in the reference): times the refine kernel
+ * forms (occupancy x traversal-stack LDS share, and the wavefront engine)
+ * on this scene and Params with spp_per_trial samples (1..64) into a
+ * scratch accumulator, leaves the fastest tuning set on the context and
+ * returns it (and its trial time). All forms give bit-identical images. */
+int oxpt_autotune(oxpt_ctx *ctx, const oxpt_scene *scene, const oxpt_params *params, uint32_t spp_per_trial,
+                  oxpt_tuning *best, double *best_ms);
 
 /* Debug / parity probes (no counterpart in the reference: they expose the
  * kernel's integer sampler and light-CDF picks for bit-exact checks).
@@ -337,6 +344,20 @@ void oxpt_packed_free(oxpt_packed *p);
 int oxpt_packed_words(const oxpt_packed *p, int which, const void **ptr, size_t *count);
 int oxpt_packed_params(const oxpt_packed *p, oxpt_params *out);
 int oxpt_packed_info(const oxpt_packed *p, uint32_t *atlas_side, uint32_t *atlas_layers, uint32_t *bvh_depth);
+
+/* 4-wide BVH nodes for geometry words a caller packed itself in the
+ * reference's layout (binary nodes lo = (bmin, first), hi = (bmax, count),
+ * node 0 the root, children the pair (first, first + 1) — the words of
+ * src/pathtrace.rs:972-985): 32 words (8 float4) per wide node, to be
+ * appended to geom at a float4 index that is a multiple of 8, with
+ * Params.offs2[3] set to that index (0 keeps the binary walk). No
+ * counterpart in the reference; for `-sys` callers that build their own
+ * upload. Two-call: out = NULL returns the node count in *out_vec4.
+ * Returns OXPT_E_INVALID when the tree exceeds the wide walk's limits
+ * (leaves over 30 slots, 2^27 slots or nodes, depth over 32); the binary
+ * walk then stays in use. */
+int oxpt_build_wide_bvh(const float *node_words, uint32_t node_count, float *out, size_t out_vec4_cap,
+                        size_t *out_vec4, uint32_t *wide_depth);
 
 /* Serving (continuous batching). A pool of prepared scenes; render requests
  * name a pool scene and options, refine requests add samples to a finished

@@ -63,7 +63,7 @@ struct oxpt_ctx {
     std::string summary;
     uint64_t budget = 0, in_use = 0;
     uint32_t n_cu = 0;
-    oxpt_tuning tuning{256, 32, 1u << 27, 0, 0, 0, 0, 0};
+    oxpt_tuning tuning{256, 32, 1u << 30, 0, 0, 0, 0, 0};
     uint32_t *d_work = nullptr;
     oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[job_cap]
     uint32_t *d_counters = nullptr;  // job_cap counters + 1 count word
@@ -866,6 +866,69 @@ int oxpt_finish(oxpt_ctx *c) {
     if (setdev(c)) return OXPT_E_BACKEND;
     hipError_t e = hipStreamSynchronize(c->stream);
     return e == hipSuccess ? OXPT_OK : hip_fail(c, "finish", e);
+}
+
+int oxpt_autotune(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, uint32_t spp_per_trial,
+                  oxpt_tuning *best_out, double *best_ms) {
+    if (!c || !s || !P || s->ctx != c) return OXPT_E_ARG;
+    if (int rc = check_params(c, s, P)) return rc;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    const uint32_t spp = std::max(1u, std::min(spp_per_trial, 64u));
+    // Candidates: megakernel occupancy forms x traversal-stack LDS share,
+    // and the wavefront engine's default form. Every one of them gives the
+    // same images, so the choice is purely a matter of speed.
+    struct Cand {
+        uint32_t kernel, stack_lds, wf_variant;
+    };
+    const Cand cands[] = {{0, 0, 0}, {3, 0, 0}, {2, 0, 0}, {5, 0, 0}, {0, 8, 0}, {3, 8, 0}, {0, OXPT_STACK_PRIVATE, 0},
+                          {7, 0, 0}};
+    oxpt_accum *tmp = nullptr;
+    if (int rc = oxpt_accum_create(c, P->dims[0], P->dims[1], &tmp)) return rc;
+    const oxpt_tuning saved = c->tuning;
+    oxpt_tuning best = saved;
+    double best_t = 1e300;
+    int rc = OXPT_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(c, OXPT_E_BACKEND, "hip: events");
+    for (const Cand &cd : cands) {
+        if (rc) break;
+        oxpt_tuning t = saved;
+        t.kernel = cd.kernel;
+        t.stack_lds = cd.stack_lds;
+        t.wf_variant = cd.wf_variant;
+        if ((rc = oxpt_ctx_set_tuning(c, &t))) break;
+        // One warm trial (first-launch costs, caches), then the fastest of
+        // two timed ones.
+        if ((rc = oxpt_refine(c, s, tmp, P, 0, spp))) break;
+        float cand_ms = 1e30f;
+        hipError_t e = hipSuccess;
+        for (uint32_t rep = 0; rep < 2 && e == hipSuccess && rc == OXPT_OK; rep++) {
+            e = hipEventRecord(e0, c->stream);
+            if (e == hipSuccess && (rc = oxpt_refine(c, s, tmp, P, spp * (1 + rep), spp))) break;
+            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess) cand_ms = std::min(cand_ms, ms);
+        }
+        if (rc) break;
+        if (e != hipSuccess) {
+            rc = hip_fail(c, "autotune", e);
+            break;
+        }
+        if (cand_ms < best_t) {
+            best_t = cand_ms;
+            best = c->tuning;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    oxpt_accum_destroy(tmp);
+    c->tuning = rc ? saved : best;
+    if (rc) return rc;
+    if (best_out) *best_out = best;
+    if (best_ms) *best_ms = best_t;
+    return OXPT_OK;
 }
 
 int oxpt_ctx_timing(oxpt_ctx *c, int reset, uint64_t *launches, double *total_ms) {

@@ -90,26 +90,18 @@ template <int Waves>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves))) wf_shade(WfArgs A) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
-    if (blockIdx.x == 0 && threadIdx.x == 0) A.live[(A.iter + 1u) & (WF_LIVE_RING - 1u)] = 0;
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in_range = slot < A.n_slots;
-    // An idle slot that stays idle is neither read in full nor written back,
-    // and once every item is claimed no wave touches the item counter: the
-    // last iterations of a pass, with few live paths, stay cheap.
-    const uint32_t item0 = in_range ? A.slots[slot].item : NONE;
-    const bool was_busy = item0 != NONE;
-    // A stale (smaller) counter only costs a wasted claim: it never grows back.
-    const bool items_left = *(volatile const uint32_t *)A.item_counter < A.n_items;
-    if (!__syncthreads_or(was_busy || (in_range && items_left))) return;
     for (uint32_t i = threadIdx.x; i < OXPT_SOBOL_WORDS; i += blockDim.x)
         lds_sobol[i] = A.jobs[0].tables[A.jobs[0].P.offs0[0] + i];
     for (uint32_t i = threadIdx.x; i < OXPT_SHEEN_WORDS; i += blockDim.x)
         lds_sheen[i] = bitsf(A.jobs[0].tables[A.jobs[0].P.offs0[1] + i]);
     __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.live[(A.iter + 1u) & (WF_LIVE_RING - 1u)] = 0;
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in_range = slot < A.n_slots;
     PathSlot st;
-    if (was_busy) st = A.slots[slot];
+    if (in_range) st = A.slots[slot];
     else st.item = NONE;
-    bool busy = was_busy;
+    bool busy = in_range && st.item != NONE;
     if (busy) {
         const Job &J = A.jobs[st.job];
         const oxpt_params &P = J.P;
@@ -390,7 +382,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves)
     }
     // 6. Empty slots claim the next items: one atomic per wave.
     const uint32_t lane = __lane_id();
-    bool want = in_range && !busy && items_left;
+    bool want = in_range && !busy;
     uint64_t m = __ballot(want);
     if (m) {
         uint32_t leader = __ffsll((unsigned long long)m) - 1u;
@@ -448,7 +440,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves)
             }
         }
     }
-    if (busy || was_busy) A.slots[slot] = st;
+    if (in_range) A.slots[slot] = st;
     uint64_t live = __ballot(busy);
     if (lane == 0 && live) atomicAdd(A.live + (A.iter & (WF_LIVE_RING - 1u)), (uint32_t)__popcll(live));
 }

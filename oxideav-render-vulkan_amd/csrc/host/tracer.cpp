@@ -417,6 +417,22 @@ int oxpt_packed_info(const oxpt_packed *p, uint32_t *atlas_side, uint32_t *atlas
     return OXPT_OK;
 }
 
+int oxpt_build_wide_bvh(const float *node_words, uint32_t node_count, float *out, size_t out_vec4_cap,
+                        size_t *out_vec4, uint32_t *wide_depth) {
+    if (!node_words || node_count == 0 || !out_vec4) return OXPT_E_ARG;
+    std::vector<uint32_t> nw((size_t)node_count * 8);
+    std::memcpy(nw.data(), node_words, nw.size() * 4);
+    std::vector<float> wide;
+    uint32_t depth = oxpt::build_wide(nw, wide);
+    if (wide.empty()) return OXPT_E_INVALID;
+    *out_vec4 = wide.size() / 4;
+    if (wide_depth) *wide_depth = depth;
+    if (!out) return OXPT_OK;
+    if (out_vec4_cap < wide.size() / 4) return OXPT_E_ARG;
+    std::memcpy(out, wide.data(), wide.size() * 4);
+    return OXPT_OK;
+}
+
 void oxpt_abi_sizes(uint32_t out[11]) {
     const uint32_t v[11] = {sizeof(oxpt_params), sizeof(oxpt_frame), sizeof(oxpt_resolve_params),
                             sizeof(oxpt_scene_desc), sizeof(oxpt_tex_binding), sizeof(oxpt_material),

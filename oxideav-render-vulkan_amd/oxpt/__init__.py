@@ -190,9 +190,11 @@ _sig("oxpt_ctx_timing", C.c_int, _vp, C.c_int, P(u64), P(C.c_double))
 _sig("oxpt_refine_batch", C.c_int, _vp, P(BatchItem), sz)
 _sig("oxpt_ctx_set_tuning", C.c_int, _vp, P(Tuning))
 _sig("oxpt_ctx_get_tuning", C.c_int, _vp, P(Tuning))
+_sig("oxpt_autotune", C.c_int, _vp, _vp, P(Params), u32, P(Tuning), P(C.c_double))
 _sig("oxpt_debug_pattern_words", C.c_int, _vp, _vp, u32, sz, P(u32), P(u32), P(u32), P(u32), P(u32))
 _sig("oxpt_debug_light_picks", C.c_int, _vp, _vp, P(Params), sz, P(f32), P(u32))
 _sig("oxpt_debug_closest_hit", C.c_int, _vp, _vp, P(Params), sz, P(f32), P(f32), P(u32))
+_sig("oxpt_build_wide_bvh", C.c_int, P(f32), u32, P(f32), sz, P(sz), P(u32))
 # Layer 2.
 _sig("oxpt_render_options_default", None, P(COptions))
 _sig("oxpt_material_default", None, P(Material))
@@ -587,6 +589,16 @@ class GpuPathTracer:
         lib.oxpt_ctx_get_tuning(self.ctx, C.byref(t))
         return t
 
+    def autotune(self, spp_per_trial: int = 4) -> Tuple[Tuning, float]:
+        """Pick the fastest kernel form for the synced scene (oxpt_autotune);
+        valid after sync + one refine. Images do not change."""
+        t, ms = Tuning(), C.c_double()
+        P = self.params()
+        rc = lib.oxpt_autotune(self.ctx, self.device_scene(), C.byref(P), spp_per_trial, C.byref(t), C.byref(ms))
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        return t, ms.value
+
     def debug_pattern_words(self, seed, x, y, index, p) -> np.ndarray:
         arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (x, y, index, p)]
         n = arrs[0].size
@@ -855,3 +867,19 @@ def gpu_available() -> bool:
         return False
     lib.oxpt_ctx_destroy(h.value)
     return True
+
+
+def build_wide_bvh(node_words: np.ndarray) -> Tuple[np.ndarray, int]:
+    """4-wide nodes (float32, 32 per node) for binary node words in the
+    reference layout (8 words per node); see oxpt_build_wide_bvh."""
+    nw = np.ascontiguousarray(node_words, dtype=np.float32)
+    n = nw.size // 8
+    cnt, depth = sz(), u32()
+    rc = lib.oxpt_build_wide_bvh(_ptr(nw, f32), n, None, 0, C.byref(cnt), C.byref(depth))
+    if rc:
+        _raise(rc, "build_wide_bvh")
+    out = np.zeros(cnt.value * 4, dtype=np.float32)
+    rc = lib.oxpt_build_wide_bvh(_ptr(nw, f32), n, _ptr(out, f32), cnt.value, C.byref(cnt), C.byref(depth))
+    if rc:
+        _raise(rc, "build_wide_bvh")
+    return out, depth.value

@@ -3,6 +3,8 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
+
 import oxpt
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -52,3 +54,16 @@ def test_missing_device_is_a_backend_error():
 
     with pytest.raises(oxpt.BackendError):
         oxpt.GpuPathTracer(0)
+
+
+def test_build_wide_bvh_matches_the_packed_scene():
+    """oxpt_build_wide_bvh over the binary node words reproduces the 4-wide
+    nodes the packer appends (for callers that pack geom themselves)."""
+    scene = oxpt.Scene.terrain(seed=4, gx=40, gz=20, n_spheres=6, subdiv=2, n_emissive=8)
+    pk = oxpt.Packed(scene, oxpt.RenderOptions(width=8, height=8))
+    P = pk.params
+    nodes = pk.geom[: 8 * P.env[3]]
+    wide, depth = oxpt.build_wide_bvh(nodes)
+    assert depth >= 1
+    assert np.array_equal(wide.view(np.uint32), pk.geom[4 * P.offs2[3]: 4 * P.offs2[3] + wide.size].view(np.uint32))
+    assert 4 * P.offs2[3] + wide.size == pk.geom.size

@@ -233,3 +233,19 @@ def test_layered_texture_scene_parity(gpu):
     _, ci, ch = oracle_frame(scene, opts)
     p = parity("layered", gh, gi, ch, ci, 60.0)
     print(f"layered: PSNR {p:.2f} dB")
+
+
+def test_autotune_keeps_the_image(gpu):
+    """The autotuner picks one of the kernel forms; the image does not change."""
+    scene = small_terrain()
+    opts = oxpt.RenderOptions(width=96, height=64, samples_per_pixel=4, max_bounces=6)
+    gpu.set_environment(None)
+    gpu.set_tuning(kernel=0)
+    before = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+    t, ms = gpu.autotune(2)
+    assert ms > 0 and t.kernel in (0, 2, 3, 5, 7)
+    gpu.reset()
+    gpu.refine(opts.samples_per_pixel)
+    after = gpu.hdr().view(np.uint32)
+    gpu.set_tuning(kernel=0)
+    assert np.array_equal(before, after), f"tuning {t.kernel}/{t.stack_lds}"
