@@ -46,10 +46,31 @@ __device__ __forceinline__ Scene make_scene(const oxpt_params &P, const float4 *
     S.env_rad = P.offs2[1];
     S.p_env = P.fparams[2];
     S.lds_sheen = lds_sheen;
+    S.lds_marg = nullptr;
+    S.lds_ecdf = nullptr;
     return S;
 }
 __device__ __forceinline__ Scene make_scene(const TraceArgs &A, const float *lds_sheen) {
     return make_scene(A.P, A.geom, A.attrs, A.tables, A.atlas, A.atlas_side, lds_sheen);
+}
+
+// LDS copies of the light-selection CDFs for single-scene kernels: the
+// environment marginal (up to 1024 words) and the emissive CDF (up to 64
+// records); larger tables are read from global memory. Every workgroup
+// stages them once, after which the binary searches run on LDS.
+constexpr uint32_t LDS_MARG_WORDS = 1024, LDS_ECDF_WORDS = 64;
+__device__ __forceinline__ void stage_light_cdfs(const uint32_t *tables, const oxpt_params &P, float *lds_cdf,
+                                                 Scene &S) {
+    const bool marg = P.env[2] != 0 && P.env[1] + 1u <= LDS_MARG_WORDS;
+    const bool ecdf = P.cfg[3] != 0 && P.cfg[3] <= LDS_ECDF_WORDS;
+    if (marg)
+        for (uint32_t i = threadIdx.x; i <= P.env[1]; i += blockDim.x) lds_cdf[i] = bitsf(tables[P.offs1[2] + i]);
+    if (ecdf)
+        for (uint32_t i = threadIdx.x; i < P.cfg[3]; i += blockDim.x)
+            lds_cdf[LDS_MARG_WORDS + i] = bitsf(tables[P.offs1[0] + 4u * i]);
+    __syncthreads();
+    S.lds_marg = marg ? lds_cdf : nullptr;
+    S.lds_ecdf = ecdf ? lds_cdf + LDS_MARG_WORDS : nullptr;
 }
 
 __device__ __forceinline__ F3 clamp_c(float cl, F3 c) {
@@ -252,7 +273,8 @@ __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *ld
                 F3 a = tri_pos(S, lslot, 0), b = tri_pos(S, lslot, 1), c = tri_pos(S, lslot, 2);
                 bool has_target = false;
                 F3 q = f3(0.0f);
-                if (triangle_solid_angle(st.p, a, b, c) >= MIN_SPHERICAL_SOLID_ANGLE) {
+                const float omega = triangle_solid_angle(st.p, a, b, c);
+                if (omega >= MIN_SPHERICAL_SOLID_ANGLE) {
                     F3 sl = sample_spherical_triangle(st.p, a, b, c, st.pl_u.x, st.pl_u.y);
                     if (dot(sl, sl) > 0.0f) {
                         F3 n = cross(b - a, c - a);
@@ -287,7 +309,7 @@ __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *ld
                         float dist = sqrtf(dist2);
                         F3 ld = to / dist;
                         float cos_l = -dot(lng, ld);
-                        float tp = tri_pdf(S, li, lslot, st.p, q);
+                        float tp = tri_pdf_omega(S, li, a, b, c, omega, st.p, q);
                         if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && tp > 0.0f && finite(tp)) {
                             ok = true;
                             l = ld;
@@ -394,8 +416,10 @@ trace_persistent(TraceArgs A, uint32_t lds_depth) {
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
     extern __shared__ uint64_t lds_dyn[];
     lds_u64 *lds_stack = to_lds(lds_dyn) + (threadIdx.x >> 6) * lds_depth * 64u;
+    __shared__ float lds_cdf[LDS_MARG_WORDS + LDS_ECDF_WORDS];
     stage_tables(A.tables, A.P, lds_sobol, lds_sheen);
-    const Scene S = make_scene(A, lds_sheen);
+    Scene S = make_scene(A, lds_sheen);
+    stage_light_cdfs(A.tables, A.P, lds_cdf, S);
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
     const uint32_t w = A.P.dims[0], h = A.P.dims[1];
@@ -439,8 +463,14 @@ trace_persistent(TraceArgs A, uint32_t lds_depth) {
             init_path(A.P, lds_sobol, px, py, A.first + done, st);
             need_init = false;
         }
-        bool covered;
-        if (step_path(A.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) {
+        // With punctual lights a hit has many NEE shadow rays: trace them
+        // before returning to the claim step, so the lanes of a wave stay in
+        // the same phase of their bounces (less divergent shading).
+        bool covered, ended;
+        while (!(ended = step_path(A.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) && st.shadow &&
+               A.P.cfg[2] != 0) {
+        }
+        if (ended) {
             if (covered) {
                 F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
                 acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
@@ -461,8 +491,10 @@ trace_persistent(TraceArgs A, uint32_t lds_depth) {
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) trace_tiles(TraceArgs A) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
     __shared__ float lds_sheen[OXPT_SHEEN_WORDS];
+    __shared__ float lds_cdf[LDS_MARG_WORDS + LDS_ECDF_WORDS];
     stage_tables(A.tables, A.P, lds_sobol, lds_sheen);
-    const Scene S = make_scene(A, lds_sheen);
+    Scene S = make_scene(A, lds_sheen);
+    stage_light_cdfs(A.tables, A.P, lds_cdf, S);
     uint32_t px, py;
     item_pixel(A.tiles_x, blockIdx.x * 64u + threadIdx.x, px, py);
     const uint32_t w = A.P.dims[0], h = A.P.dims[1];
@@ -550,8 +582,11 @@ trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
                 break;                   // job j is drained for this wave
             }
             if (!has_path) continue;
-            bool covered;
-            if (step_path(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) {
+            bool covered, ended;
+            while (!(ended = step_path(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) && st.shadow &&
+                   J.P.cfg[2] != 0) {
+            }
+            if (ended) {
                 F3 r = covered && finite3(st.radiance) ? st.radiance : f3(0.0f);
                 B.samples[J.sample_base + item] = make_float4(r.x, r.y, r.z, covered ? 1.0f : 0.0f);
                 has_path = false;

@@ -159,6 +159,11 @@ struct Scene {
     uint32_t env_marg, env_cond, env_mass, env_rad;
     float p_env;
     const float *lds_sheen;
+    // LDS copies of the light-selection CDFs (null: read them from tables):
+    // the environment marginal (env_h + 1 floats) and the emissive CDF
+    // (one float per emissive triangle).
+    const float *lds_marg;
+    const float *lds_ecdf;
 };
 
 __device__ __forceinline__ float tf(const Scene &S, uint32_t i) { return bitsf(S.tables[i]); }

@@ -535,13 +535,24 @@ __device__ __forceinline__ uint32_t pick_elight(const Scene &S, float u) {
     uint32_t n = S.n_emissive, lo = 0, hi = n;
     while (lo < hi) {
         uint32_t mid = (lo + hi) / 2u;
-        if (tf(S, S.off_elights + 4u * mid) <= u) lo = mid + 1u;
+        const float c = S.lds_ecdf ? S.lds_ecdf[mid] : tf(S, S.off_elights + 4u * mid);
+        if (c <= u) lo = mid + 1u;
         else hi = mid;
     }
     return min(lo, n - 1u);
 }
+// Marginal CDF word y (the LDS copy when staged).
+__device__ __forceinline__ float env_marg_at(const Scene &S, uint32_t y) {
+    return S.lds_marg ? S.lds_marg[y] : tf(S, S.env_marg + y);
+}
 __device__ __forceinline__ uint32_t env_row(const Scene &S, float u0) {
-    return min(max(partition_point(S, S.env_marg, S.env_h + 1u, u0), 1u), S.env_h) - 1u;
+    uint32_t lo = 0, hi = S.env_h + 1u;  // partition_point(c <= u0), as above
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2u;
+        if (env_marg_at(S, mid) <= u0) lo = mid + 1u;
+        else hi = mid;
+    }
+    return min(max(lo, 1u), S.env_h) - 1u;
 }
 __device__ __forceinline__ uint32_t env_col(const Scene &S, uint32_t y, float u1) {
     uint32_t row = S.env_cond + y * (S.env_w + 1u);
@@ -589,10 +600,10 @@ __device__ F3 sample_spherical_triangle(F3 p, F3 a0, F3 b0, F3 c0, float u0, flo
     return nrm_any(d);
 }
 
-__device__ float tri_pdf(const Scene &S, uint32_t li, uint32_t slot, F3 p, F3 q) {
-    F3 a = tri_pos(S, slot, 0), b = tri_pos(S, slot, 1), c = tri_pos(S, slot, 2);
+// tri_pdf given the light's vertices and their solid angle from p (the NEE
+// sampler has both already).
+__device__ float tri_pdf_omega(const Scene &S, uint32_t li, F3 a, F3 b, F3 c, float omega, F3 p, F3 q) {
     float sel = (1.0f - S.p_env) * elight_pmf(S, li);
-    float omega = triangle_solid_angle(p, a, b, c);
     if (omega >= MIN_SPHERICAL_SOLID_ANGLE) return sel / omega;
     F3 cr = cross(b - a, c - a);
     float len = sqrtf(dot(cr, cr));
@@ -602,6 +613,10 @@ __device__ float tri_pdf(const Scene &S, uint32_t li, uint32_t slot, F3 p, F3 q)
     float cs = fabsf(dot(cr, to) / (len * sqrtf(d2)));
     if (cs <= 0.0f) return 0.0f;
     return sel * d2 / (0.5f * len * cs);
+}
+__device__ float tri_pdf(const Scene &S, uint32_t li, uint32_t slot, F3 p, F3 q) {
+    F3 a = tri_pos(S, slot, 0), b = tri_pos(S, slot, 1), c = tri_pos(S, slot, 2);
+    return tri_pdf_omega(S, li, a, b, c, triangle_solid_angle(p, a, b, c), p, q);
 }
 
 struct Texel {
@@ -638,11 +653,10 @@ __device__ EnvSample env_sample(const Scene &S, float u0, float u1) {
     r.le = f3(0.0f);
     r.pdf = 0.0f;
     uint32_t w = S.env_w, h = S.env_h;
-    uint32_t marg = S.env_marg;
     uint32_t y = env_row(S, u0);
     uint32_t row = S.env_cond + y * (w + 1u);
     uint32_t x = env_col(S, y, u1);
-    float my = (u0 - tf(S, marg + y)) / fmaxf(tf(S, marg + y + 1u) - tf(S, marg + y), 1.0e-12f);
+    float my = (u0 - env_marg_at(S, y)) / fmaxf(env_marg_at(S, y + 1u) - env_marg_at(S, y), 1.0e-12f);
     float mx = (u1 - tf(S, row + x)) / fmaxf(tf(S, row + x + 1u) - tf(S, row + x), 1.0e-12f);
     float u = ((float)x + clampf(mx, 0.0f, 0.99999f)) / (float)w;
     float v = ((float)y + clampf(my, 0.0f, 0.99999f)) / (float)h;

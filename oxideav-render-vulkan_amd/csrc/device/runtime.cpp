@@ -17,8 +17,10 @@
 
 namespace {
 
-// LDS taken by the Sobol' and sheen tables in every trace workgroup.
+// LDS taken by the Sobol' and sheen tables in every trace workgroup, plus
+// the light-selection CDF copies of the single-scene kernels.
 constexpr uint32_t kTableLds = 2560;
+constexpr uint32_t kCdfLds = 4352;
 
 // Waves per SIMD each megakernel form is built for (oxpt_tuning.kernel;
 // kernel 0 is 5 waves for a single refine and 3 for the batched megakernel).
@@ -43,8 +45,9 @@ uint32_t stack_lds_entries(const oxpt_tuning &t, uint32_t block, bool batched) {
     const uint32_t waves_cu = 4u * form_waves(t.kernel, batched);
     const uint32_t wg_cu = std::max(1u, waves_cu / (block / 64u));
     const uint32_t per_wg = 160u * 1024u / wg_cu;
-    if (per_wg <= kTableLds) return 0;
-    return std::min(16u, (per_wg - kTableLds) / ((block / 64u) * 512u));
+    const uint32_t fixed = kTableLds + (batched ? 0u : kCdfLds);
+    if (per_wg <= fixed) return 0;
+    return std::min(16u, (per_wg - fixed) / ((block / 64u) * 512u));
 }
 
 // Device opening is serialised process-wide (src/gpu.rs:978-992).
@@ -429,7 +432,7 @@ int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
     if (t->stack_lds != OXPT_STACK_PRIVATE && t->stack_lds > 64)
         return fail(c, OXPT_E_ARG, "tuning: stack_lds must be 0 (default), 1..64 or OXPT_STACK_PRIVATE");
     if (t->stack_lds != OXPT_STACK_PRIVATE && t->stack_lds != 0 &&
-        (c->tuning.block / 64u) * t->stack_lds * 512u + kTableLds > 65536u)
+        (c->tuning.block / 64u) * t->stack_lds * 512u + kTableLds + kCdfLds > 65536u)
         return fail(c, OXPT_E_ARG, "tuning: stack_lds does not fit in 64 KiB of LDS per workgroup");
     if (t->bvh != 0 && t->bvh != 2) return fail(c, OXPT_E_ARG, "tuning: bvh must be 0 (4-wide) or 2 (binary)");
     c->tuning.kernel = t->kernel;

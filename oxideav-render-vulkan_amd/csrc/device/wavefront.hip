@@ -58,6 +58,8 @@ __device__ __forceinline__ Scene job_scene(const Job &J, const float *lds_sheen)
     S.env_rad = J.P.offs2[1];
     S.p_env = J.P.fparams[2];
     S.lds_sheen = lds_sheen;
+    S.lds_marg = nullptr;
+    S.lds_ecdf = nullptr;
     return S;
 }
 
@@ -244,7 +246,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves)
                                 F3 a = tri_pos(S, lslot, 0), bb = tri_pos(S, lslot, 1), c = tri_pos(S, lslot, 2);
                                 bool has_target = false;
                                 F3 q = f3(0.0f);
-                                if (triangle_solid_angle(p, a, bb, c) >= MIN_SPHERICAL_SOLID_ANGLE) {
+                                const float omega = triangle_solid_angle(p, a, bb, c);
+                                if (omega >= MIN_SPHERICAL_SOLID_ANGLE) {
                                     F3 sl = sample_spherical_triangle(p, a, bb, c, pl_u.x, pl_u.y);
                                     if (dot(sl, sl) > 0.0f) {
                                         F3 n = cross(bb - a, c - a);
@@ -279,7 +282,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves)
                                         float dist = sqrtf(dist2);
                                         F3 ld = to / dist;
                                         float cos_l = -dot(lng, ld);
-                                        float tp = tri_pdf(S, li, lslot, p, q);
+                                        float tp = tri_pdf_omega(S, li, a, bb, c, omega, p, q);
                                         if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && tp > 0.0f && finite(tp)) {
                                             ok = true;
                                             l = ld;

@@ -11,6 +11,7 @@ import pytest
 import oracle
 import oxpt
 from ptutil import oracle_frame, parity, psnr
+import ptscenes
 
 pytestmark = pytest.mark.gpu
 
@@ -174,6 +175,21 @@ def test_plain_kernel_matches_persistent(gpu):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_punctual_lights_forms_bitwise(gpu):
+    """With punctual lights the persistent forms trace a hit's shadow rays
+    before claiming new work; images stay bitwise those of the plain form."""
+    scene = ptscenes.materials_scene()
+    opts = oxpt.RenderOptions(width=64, height=48, samples_per_pixel=6, max_bounces=8)
+    gpu.set_environment(None)
+    gpu.set_tuning(kernel=1)
+    ref = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+    for kernel in (0, 3, 7):
+        gpu.set_tuning(kernel=kernel)
+        got = gpu.render_hdr(scene, opts).view(np.uint32)
+        assert np.array_equal(got, ref), f"kernel {kernel}"
+    gpu.set_tuning(kernel=0)
+
+
 def test_kernel_forms_and_stack_split_bitwise(gpu):
     """Every occupancy form and every LDS/private split of the traversal stack
     (including a 1-entry LDS share, so deep pushes land in private memory)
@@ -205,7 +221,6 @@ def test_tuning_rejects_bad_stack_split(gpu):
 # punctual lights, alpha modes, textures with mips, direct-only.
 # ---------------------------------------------------------------------------
 
-import ptscenes  # noqa: E402
 
 
 @pytest.mark.parametrize("name,bounces,spp", [("materials", 8, 32), ("alpha", 8, 32), ("checker", 4, 16),
