@@ -305,6 +305,7 @@ def main():
         log(f"step {i + 1}/{args.steps}: {lat[-1] * 1e3:.1f} ms")
     t1 = time.perf_counter()
     launches, kern_ms = pt.timing()
+    kernel_name = pt.last_kernel()
     w, h = opts.width, opts.height
     frame_paths = w * h * spp
     total_paths = frame_paths * args.steps
@@ -359,7 +360,7 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": pmc_traffic(args, per_launch_paths),
-            "kernel": "trace_persistent" if args.kernel == 0 else "trace_tiles",
+            "kernel": kernel_name,
             "bytes_per_path": round(bytes_per_path, 1), "paths_per_launch": int(per_launch_paths),
             "avg_launch_ms": round(avg_launch_s * 1e3, 3),
         }
@@ -385,7 +386,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene generators)",
         "config": {"workload": desc, "width": w, "height": h, "spp": spp, "max_bounces": opts.max_bounces,
-                   "triangles": scene.triangle_count(), "kernel": "persistent" if args.kernel == 0 else "plain"},
+                   "triangles": scene.triangle_count(), "kernel": kernel_name},
         "latency_ms": {"p50": pct(50), "p99": pct(99)},
         "psnr_db_vs_cpu": None if psnr_db is None else round(psnr_db, 2),
         "roofline": roofline,

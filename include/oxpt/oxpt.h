@@ -70,6 +70,9 @@ void oxpt_ctx_destroy(oxpt_ctx *ctx);
 const char *oxpt_last_error(const oxpt_ctx *ctx);
 /* "<device name> (gfx arch, HIP)" — adapter_summary (src/pathtrace.rs:464-467). */
 const char *oxpt_ctx_summary(const oxpt_ctx *ctx);
+/* Name of the trace kernel the last refine / refine_batch launched (for
+ * profiling and reports; no counterpart in the reference). */
+const char *oxpt_ctx_last_kernel(const oxpt_ctx *ctx);
 /* Device memory budget in bytes that uploads and accumulators may use
  * (default: the device's total memory); allocation past it fails with
  * OXPT_E_BACKEND, as check_size does (src/pathtrace.rs:567-576). */

@@ -74,6 +74,7 @@ struct oxpt_ctx {
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;  // captured batched step (reset + trace)
     uint32_t graph_block = 0, graph_grid = 0, graph_kernel = 0, graph_lds = 0;
+    const char *last_kernel = "";  // name of the last trace kernel launched
     float4 *d_samples = nullptr;  // batched launch: one (rgb, covered) per path of a round
     size_t samples_bytes = 0;
     uint64_t graph_launches = 0;
@@ -412,6 +413,7 @@ void oxpt_ctx_destroy(oxpt_ctx *c) {
 
 const char *oxpt_last_error(const oxpt_ctx *c) { return c ? c->err.c_str() : "null context"; }
 const char *oxpt_ctx_summary(const oxpt_ctx *c) { return c ? c->summary.c_str() : ""; }
+const char *oxpt_ctx_last_kernel(const oxpt_ctx *c) { return c ? c->last_kernel : ""; }
 
 int oxpt_ctx_set_budget(oxpt_ctx *c, uint64_t bytes) {
     if (!c) return OXPT_E_ARG;
@@ -611,6 +613,20 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     if (count == 0) return OXPT_OK;
     if (int rc = check_params(c, s, P)) return rc;
     if (setdev(c)) return OXPT_E_BACKEND;
+    // A frame with fewer pixels than the GPU holds resident lanes would
+    // leave most lanes idle with one pixel per lane: the default form then
+    // runs as a one-job batch, whose work item is a single path (each
+    // pixel's samples are still added in order, so the image is the same).
+    const uint64_t resident_lanes = (uint64_t)c->n_cu * 4u * form_waves(0, false) * 64u;
+    if (c->tuning.kernel == 0 && (uint64_t)a->w * a->h < resident_lanes && count > 1) {
+        oxpt_batch_item it;
+        it.scene = s;
+        it.accum = a;
+        it.params = *P;
+        it.first = first;
+        it.count = count;
+        return oxpt_refine_batch(c, &it, 1);
+    }
     if (c->tuning.kernel == 7) {
         oxd::Job J;
         std::memset(&J, 0, sizeof J);
@@ -629,6 +645,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
+        c->last_kernel = "wavefront";
         int rc = wf_run(c, {J});
         if (timed) {
             (void)hipEventRecord(ep.b, c->stream);
@@ -651,6 +668,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
+        c->last_kernel = c->tuning.kernel == 1 ? "trace_tiles" : "trace_persistent";
         hipError_t e = oxd::launch_trace(args, c->tuning.kernel, block, grid, lds, c->stream);
         if (e != hipSuccess) return hip_fail(c, "trace launch", e);
         if (timed) {
@@ -732,6 +750,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
+        c->last_kernel = "wavefront";
         int rc = wf_run(c, jobs);
         if (timed) {
             (void)hipEventRecord(ep.b, c->stream);
@@ -818,6 +837,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
             e = hipMemcpyAsync(c->d_counters + c->job_cap, &nj, 4, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch job upload", e);
         oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap, c->d_samples};
+        c->last_kernel = "trace_batch";
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);

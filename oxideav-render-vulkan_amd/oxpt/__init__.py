@@ -172,6 +172,7 @@ _sig("oxpt_ctx_create", C.c_int, C.c_int, P(_vp))
 _sig("oxpt_ctx_destroy", None, _vp)
 _sig("oxpt_last_error", C.c_char_p, _vp)
 _sig("oxpt_ctx_summary", C.c_char_p, _vp)
+_sig("oxpt_ctx_last_kernel", C.c_char_p, _vp)
 _sig("oxpt_ctx_set_budget", C.c_int, _vp, u64)
 _sig("oxpt_ctx_bytes_in_use", u64, _vp)
 _sig("oxpt_scene_upload", C.c_int, _vp, P(SceneDesc), P(_vp))
@@ -577,6 +578,10 @@ class GpuPathTracer:
         n, ms = u64(), C.c_double()
         lib.oxpt_ctx_timing(self.ctx, 1 if reset else 0, C.byref(n), C.byref(ms))
         return n.value, ms.value
+
+    def last_kernel(self) -> str:
+        """Name of the trace kernel the last refine launched."""
+        return lib.oxpt_ctx_last_kernel(self.ctx).decode()
 
     def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0) -> None:
         t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh)
