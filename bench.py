@@ -71,12 +71,15 @@ def pmc_traffic(args, paths_per_launch):
     launch, each scaled by its reading on a plain copy of known size
     (profiles/counter_calibration_copy_1GiB.md). None for other configs."""
     path = os.path.join(ROOT, "profiles", "trace_persistent_c1_pmc.json")
+    kernel = "trace_persistent<4>"
     if args.workload != "c1" or args.kernel != 0 or args.spp or args.block or args.waves_per_cu:
         return None
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
+    if d.get("kernel") != kernel:
+        return None
     per_path = (d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"]) * 1024.0 / d["paths_per_dispatch"]
     return int(round(per_path * paths_per_launch))
 
@@ -264,8 +267,8 @@ def main():
         log(f"copied {args.copy_calibration} MiB ({n * 4} bytes read, {n * 4} bytes written)")
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the path shards by sample range or scene, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard by sample range or by scene)", file=sys.stderr)
         sys.exit(2)
 
     import oxpt

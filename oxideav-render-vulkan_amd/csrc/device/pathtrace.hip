@@ -746,7 +746,7 @@ __global__ void debug_closest_hit(TraceArgs A, uint32_t n, const float *o3, cons
 
 namespace oxd {
 
-// Kernel forms (oxpt_tuning.kernel): 0 the default (5 waves per SIMD),
+// Kernel forms (oxpt_tuning.kernel): 0 the default (4 waves per SIMD),
 // 1 plain per-pixel, 2..6 persistent built for 3, 4, 5, 6, 8 waves per SIMD,
 // 8 persistent built for 2 waves per SIMD (256 VGPRs).
 hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, uint32_t lds_depth,
@@ -762,11 +762,11 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
     switch (kernel) {
         case 2: hipLaunchKernelGGL(trace_persistent<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        case 3: hipLaunchKernelGGL(trace_persistent<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 4: hipLaunchKernelGGL(trace_persistent<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 5: hipLaunchKernelGGL(trace_persistent<6>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 6: hipLaunchKernelGGL(trace_persistent<8>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 8: hipLaunchKernelGGL(trace_persistent<2>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
-        default: hipLaunchKernelGGL(trace_persistent<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        default: hipLaunchKernelGGL(trace_persistent<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
     }
     return hipGetLastError();
 }

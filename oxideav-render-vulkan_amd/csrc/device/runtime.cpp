@@ -23,10 +23,10 @@ constexpr uint32_t kTableLds = 2560;
 constexpr uint32_t kCdfLds = 4352;
 
 // Waves per SIMD each megakernel form is built for (oxpt_tuning.kernel;
-// kernel 0 is 5 waves for a single refine and 3 for the batched megakernel).
+// kernel 0 is 4 waves for a single refine and 3 for the batched megakernel).
 uint32_t form_waves(uint32_t kernel, bool batched) {
     switch (kernel) {
-        case 0: return batched ? 3 : 5;
+        case 0: return batched ? 3 : 4;
         case 2: return 3;
         case 3: return 4;
         case 4: return 5;
