@@ -152,13 +152,17 @@ typedef struct oxpt_tuning {
     uint32_t stack_lds;      /* megakernel traversal-stack entries per lane kept in LDS (the rest in private
                                 memory): 0 = default for the form, 1..64, or OXPT_STACK_PRIVATE */
     uint32_t bvh;            /* 0 = walk the scene's 4-wide tree when it has one, 2 = the binary tree */
-    uint32_t reserved;
+    uint32_t tail;           /* persistent megakernel: the last tail/16 of the frame's 8x8 tiles are traced
+                                one path per claim (then added in sample order; at most 2^28 buffered
+                                samples per launch): 0 = default (8, half the frame), 1..16, or
+                                OXPT_TAIL_NONE (every pixel as one run of samples) */
 } oxpt_tuning;
 #define OXPT_STACK_PRIVATE 0xFFFFu
+#define OXPT_TAIL_NONE 0xFFFFu
 int oxpt_ctx_set_tuning(oxpt_ctx *ctx, const oxpt_tuning *t);
 int oxpt_ctx_get_tuning(const oxpt_ctx *ctx, oxpt_tuning *t);
 /* Autotuner (no counterpart in the reference): times the refine kernel
- * forms (occupancy x traversal-stack LDS share, and the wavefront engine)
+ * forms (occupancy x traversal-stack LDS share x tail share, and the wavefront engine)
  * on this scene and Params with spp_per_trial samples (1..64) into a
  * scratch accumulator, leaves the fastest tuning set on the context and
  * returns it (and its trial time). All forms give bit-identical images. */

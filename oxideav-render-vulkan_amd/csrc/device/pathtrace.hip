@@ -11,7 +11,11 @@
 //   counted with a ballot, one lane claims that many pixels with a single
 //   atomic on the work counter, and each idle lane takes its rank among
 //   them. Work items are ordered in 8x8 pixel tiles so a fresh wave starts
-//   on one coherent tile.
+//   on one coherent tile. The last tiles of a launch (the tail) are claimed
+//   one path at a time instead, each path's (rgb, covered) going to a sample
+//   buffer that tail_reduce adds into the accumulator in sample order: lanes
+//   out of pixels take tail paths rather than idling while the rest of the
+//   wave finishes whole pixels, and the sums are the same additions.
 // trace_tiles — the plain form: one thread per pixel, 8x8 tiles per wave,
 //   no regeneration (the reference's dispatch shape).
 // resolve_pt — accumulator -> RGBA8 / RGBA32F (src/shaders/resolve.wgsl:151-188).
@@ -423,37 +427,63 @@ trace_persistent(TraceArgs A, uint32_t lds_depth) {
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
     const uint32_t w = A.P.dims[0], h = A.P.dims[1];
-    bool has_pixel = false, exhausted = false, need_init = false;
-    uint32_t px = 0, py = 0, done = 0;
+    // A lane holds samples [done, run_end) of pixel (px, py): a pixel's
+    // whole run added into `acc`, or (to_buffer) one tail path whose sample
+    // goes to the sample buffer at slot_base + 64 * sample.
+    bool has_pixel = false, exhausted = false, need_init = false, to_buffer = false;
+    uint32_t px = 0, py = 0, done = 0, run_end = 0, slot_base = 0;
+    // Wave-uniform: the pixel queue is spent, claims go to the tail paths.
+    bool pixels_spent = A.split == 0;
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     PathState st;
     for (;;) {
-        // Regenerate: idle lanes claim pixels with one atomic per wave.
+        // Regenerate: idle lanes claim work with one atomic per wave — whole
+        // pixels while items before the split remain, then single tail
+        // paths (a wave's lanes take one tile at one sample when they claim
+        // together).
         bool want = !has_pixel && !exhausted;
         uint64_t m = __ballot(want);
         if (m) {
-            uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+            const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+            const uint32_t n = (uint32_t)__popcll(m);
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            if (want) {
-                for (;;) {
-                    uint32_t item = base + (uint32_t)__popcll(m & lt_mask);
-                    if (item >= A.n_items) {
-                        exhausted = true;
-                        break;
+            if (lane == leader) base = atomicAdd(A.work + (pixels_spent ? 1u : 0u), n);
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, leader));
+            const uint32_t rank = (uint32_t)__popcll(m & lt_mask);
+            if (!pixels_spent) {
+                if (want) {
+                    const uint32_t item = base + rank;
+                    // Items outside the image (ragged edge tiles) have no
+                    // pixel: the lane claims again on the next ballot.
+                    if (item < A.split) {
+                        item_pixel(A.tiles_x, item, px, py);
+                        if (px < w && py < h) {
+                            has_pixel = true;
+                            to_buffer = false;
+                            done = 0;
+                            run_end = A.count;
+                            need_init = true;
+                            acc = A.accum[(size_t)py * w + px];
+                        }
                     }
-                    item_pixel(A.tiles_x, item, px, py);
+                }
+                if (base + n >= A.split) pixels_spent = true;
+            } else if (want) {
+                const uint32_t q = base + rank;
+                if (q >= A.tail_work) {
+                    exhausted = true;
+                } else {
+                    const uint32_t tile = q / (64u * A.count), rem = q - tile * 64u * A.count;
+                    const uint32_t in = rem & 63u;
+                    item_pixel(A.tiles_x, A.split + tile * 64u + in, px, py);
                     if (px < w && py < h) {
                         has_pixel = true;
-                        done = 0;
+                        to_buffer = true;
+                        done = rem >> 6;
+                        run_end = done + 1u;
+                        slot_base = tile * 64u * A.count + in;
                         need_init = true;
-                        acc = A.accum[(size_t)py * w + px];
-                        break;
                     }
-                    // Item outside the image (ragged edge tiles): no pixel,
-                    // try again on the next ballot.
-                    break;
                 }
             }
         }
@@ -471,19 +501,40 @@ trace_persistent(TraceArgs A, uint32_t lds_depth) {
                A.P.cfg[2] != 0) {
         }
         if (ended) {
-            if (covered) {
+            if (to_buffer) {
+                F3 r = covered && finite3(st.radiance) ? st.radiance : f3(0.0f);
+                A.samples[slot_base + 64u * done] = make_float4(r.x, r.y, r.z, covered ? 1.0f : 0.0f);
+            } else if (covered) {
                 F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
                 acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
             }
             done += 1;
-            if (done == A.count) {
-                A.accum[(size_t)py * w + px] = acc;
+            if (done == run_end) {
+                if (!to_buffer) A.accum[(size_t)py * w + px] = acc;
                 has_pixel = false;
             } else {
                 need_init = true;
             }
         }
     }
+}
+
+// The persistent kernel's tail: per tail pixel, its samples in sample order
+// into the accumulator (the same additions the per-pixel run makes).
+__global__ void __launch_bounds__(256) tail_reduce(TraceArgs A) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_items - A.split) return;
+    uint32_t px, py;
+    item_pixel(A.tiles_x, A.split + i, px, py);
+    if (px >= A.P.dims[0] || py >= A.P.dims[1]) return;
+    const size_t at = (size_t)py * A.P.dims[0] + px;
+    float4 acc = A.accum[at];
+    const float4 *smp = A.samples + (size_t)(i >> 6) * 64u * A.count + (i & 63u);
+    for (uint32_t s = 0; s < A.count; s++) {
+        const float4 r = smp[(size_t)s * 64u];
+        if (r.w > 0.0f) acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
+    }
+    A.accum[at] = acc;
 }
 
 // Plain form: one thread per pixel of an 8x8 tile, samples in order (built
@@ -624,8 +675,9 @@ __global__ void __launch_bounds__(256) batch_reduce(BatchArgs B, const Job *__re
     }
 }
 
+// The persistent kernel's two queue counters (pixels, tail paths).
 __global__ void reset_counter(uint32_t *c) {
-    if (threadIdx.x == 0) *c = 0;
+    if (threadIdx.x < 2) c[threadIdx.x] = 0;
 }
 
 __global__ void reset_counters(uint32_t *c, const uint32_t *n, uint32_t cap) {
@@ -767,6 +819,11 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
         case 6: hipLaunchKernelGGL(trace_persistent<8>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 8: hipLaunchKernelGGL(trace_persistent<2>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         default: hipLaunchKernelGGL(trace_persistent<4>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+    }
+    if (args.split < args.n_items) {
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(tail_reduce, dim3((args.n_items - args.split + 255u) / 256u), dim3(256), 0, s, args);
     }
     return hipGetLastError();
 }

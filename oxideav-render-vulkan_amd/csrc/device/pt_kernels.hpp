@@ -19,7 +19,14 @@ struct TraceArgs {
     uint32_t first, count;  // sample run of this launch
     uint32_t n_items;       // 64 * number of 8x8 tiles
     uint32_t tiles_x;
-    uint32_t *work;         // work-queue counter (persistent kernel)
+    uint32_t *work;         // work-queue counters (persistent kernel): [0] pixels, [1] tail paths
+    // Persistent kernel tail: items [split, n_items) (whole tiles) are traced
+    // one path per claim, each path's (rgb, covered) into samples[tile' * 64
+    // * count + sample * 64 + in] (tile' counted from split), and tail_reduce
+    // adds them to the accumulator in sample order. split == n_items: no tail.
+    uint32_t split;
+    uint32_t tail_work;     // (n_items - split) * count
+    float4 *samples;
 };
 
 // One job of a batched launch: a resident scene, its Params, an accumulator

@@ -21,6 +21,10 @@ namespace {
 // the light-selection CDF copies of the single-scene kernels.
 constexpr uint32_t kTableLds = 2560;
 constexpr uint32_t kCdfLds = 4352;
+// Persistent kernel tail in 1/16 of the frame (oxpt_tuning.tail), and the
+// most tail samples one launch buffers (16 B each: 4 GiB).
+constexpr uint32_t kDefaultTail = 8;
+constexpr uint64_t kTailSlots = 1ull << 28;
 
 // Waves per SIMD each megakernel form is built for (oxpt_tuning.kernel;
 // kernel 0 is 4 waves for a single refine and 3 for the batched megakernel).
@@ -189,6 +193,7 @@ oxd::TraceArgs trace_args(const oxpt_scene *s, oxpt_accum *acc, const oxpt_param
         uint32_t tx = (acc->w + 7u) / 8u, ty = (acc->h + 7u) / 8u;
         a.tiles_x = tx;
         a.n_items = 64u * tx * ty;
+        a.split = a.n_items;  // no tail unless the launch sets one
     }
     a.work = s->ctx->d_work;
     if (s->ctx->tuning.bvh == 2) a.P.offs2[3] = 0;  // walk the binary tree
@@ -437,10 +442,13 @@ int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
         (c->tuning.block / 64u) * t->stack_lds * 512u + kTableLds + kCdfLds > 65536u)
         return fail(c, OXPT_E_ARG, "tuning: stack_lds does not fit in 64 KiB of LDS per workgroup");
     if (t->bvh != 0 && t->bvh != 2) return fail(c, OXPT_E_ARG, "tuning: bvh must be 0 (4-wide) or 2 (binary)");
+    if (t->tail > 16 && t->tail != OXPT_TAIL_NONE)
+        return fail(c, OXPT_E_ARG, "tuning: tail must be 0 (default), 1..16 or OXPT_TAIL_NONE");
     c->tuning.kernel = t->kernel;
     c->tuning.wf_variant = t->wf_variant;
     c->tuning.stack_lds = t->stack_lds;
     c->tuning.bvh = t->bvh;
+    c->tuning.tail = t->tail;
     return OXPT_OK;
 }
 int oxpt_ctx_set_graph(oxpt_ctx *c, int enable) {
@@ -607,6 +615,26 @@ int oxpt_accum_write(oxpt_ctx *c, oxpt_accum *a, const float *host, size_t bytes
     return e == hipSuccess ? OXPT_OK : hip_fail(c, "accumulator write", e);
 }
 
+// Grows the per-path sample buffer (batched launches and the persistent
+// kernel's tail) to at least `slots` float4s. A captured graph refers to the
+// old buffer, so it is dropped.
+static int ensure_samples(oxpt_ctx *c, uint64_t slots) {
+    if (c->samples_bytes >= slots * 16) return OXPT_OK;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    dfree(c, c->d_samples, c->samples_bytes);
+    c->d_samples = nullptr;
+    c->samples_bytes = 0;
+    void *p = nullptr;
+    if (int rc = dalloc(c, slots * 16, &p, "per-path sample buffer")) return rc;
+    c->d_samples = (float4 *)p;
+    c->samples_bytes = slots * 16;
+    return OXPT_OK;
+}
+
 int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_params *P, uint32_t first, uint32_t count) {
     if (!c || !s || !a || !P || s->ctx != c || a->ctx != c) return OXPT_E_ARG;
     if (P->dims[0] != a->w || P->dims[1] != a->h) return fail(c, OXPT_E_ARG, "refine: params size differs from the accumulator");
@@ -659,12 +687,30 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
     const uint32_t lds = stack_lds_entries(c->tuning, block, false);
+    // Persistent forms: the last tail/16 of the frame's tiles are traced one
+    // path per claim (a wave's lanes on one tile at one sample, so the wave
+    // stays coherent), and lanes that run out of pixels take tail paths
+    // instead of idling while the rest of their wave finishes whole runs.
+    const bool persistent = c->tuning.kernel != 1;
+    const uint32_t tail16 = c->tuning.tail == 0 ? kDefaultTail : (c->tuning.tail == OXPT_TAIL_NONE ? 0u : c->tuning.tail);
     uint64_t end = (uint64_t)first + count;
     for (uint64_t f = first; f < end;) {
         uint32_t n = (uint32_t)std::min<uint64_t>(per, end - f);
         oxd::TraceArgs args = trace_args(s, a, P);
         args.first = (uint32_t)f;
         args.count = n;
+        if (persistent && n > 1 && tail16 > 0) {
+            const uint64_t tiles = args.n_items / 64u;
+            uint64_t tail_tiles = (tiles * tail16 + 15u) / 16u;
+            tail_tiles = std::min<uint64_t>(tail_tiles, kTailSlots / 64u / n);
+            const uint64_t tail_items = tail_tiles * 64u;
+            if (tail_items > 0) {
+                if (int rc = ensure_samples(c, tail_items * n)) return rc;
+                args.split = args.n_items - (uint32_t)tail_items;
+                args.tail_work = (uint32_t)(tail_items * n);
+                args.samples = c->d_samples;
+            }
+        }
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
@@ -779,20 +825,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         slots += 64u * tiles * std::min(spd, items[i].count);
     }
     if (slots >= (1ull << 32)) return fail(c, OXPT_E_ARG, "batch: round too large (lower max_paths)");
-    if (c->samples_bytes < slots * 16) {
-        (void)hipStreamSynchronize(c->stream);
-        if (c->graph) {
-            (void)hipGraphExecDestroy(c->graph);
-            c->graph = nullptr;
-        }
-        dfree(c, c->d_samples, c->samples_bytes);
-        c->d_samples = nullptr;
-        c->samples_bytes = 0;
-        void *p = nullptr;
-        if (int rc = dalloc(c, slots * 16, &p, "batch sample buffer")) return rc;
-        c->d_samples = (float4 *)p;
-        c->samples_bytes = slots * 16;
-    }
+    if (int rc = ensure_samples(c, slots)) return rc;
     std::vector<oxd::Job> jobs;
     std::vector<double> work;
     for (uint32_t done = 0; done < max_count; done += spd) {
@@ -897,14 +930,16 @@ int oxpt_autotune(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, uint32
     if (int rc = check_params(c, s, P)) return rc;
     if (setdev(c)) return OXPT_E_BACKEND;
     const uint32_t spp = std::max(1u, std::min(spp_per_trial, 64u));
-    // Candidates: megakernel occupancy forms x traversal-stack LDS share,
-    // and the wavefront engine's default form. Every one of them gives the
-    // same images, so the choice is purely a matter of speed.
+    // Candidates: megakernel occupancy forms x traversal-stack LDS share x
+    // path-granular tail share, and the wavefront engine's default form.
+    // Every one of them gives the same images, so the choice is purely a
+    // matter of speed.
     struct Cand {
-        uint32_t kernel, stack_lds, wf_variant;
+        uint32_t kernel, stack_lds, wf_variant, tail;
     };
-    const Cand cands[] = {{0, 0, 0}, {3, 0, 0}, {2, 0, 0}, {5, 0, 0}, {0, 8, 0}, {3, 8, 0}, {0, OXPT_STACK_PRIVATE, 0},
-                          {7, 0, 0}};
+    const Cand cands[] = {{0, 0, 0, 0},  {0, 0, 0, 16}, {0, 0, 0, 4},  {2, 0, 0, 0},
+                          {4, 0, 0, 0},  {0, 8, 0, 0},  {0, OXPT_STACK_PRIVATE, 0, 0},
+                          {7, 0, 0, 0}};
     oxpt_accum *tmp = nullptr;
     if (int rc = oxpt_accum_create(c, P->dims[0], P->dims[1], &tmp)) return rc;
     const oxpt_tuning saved = c->tuning;
@@ -919,6 +954,7 @@ int oxpt_autotune(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, uint32
         t.kernel = cd.kernel;
         t.stack_lds = cd.stack_lds;
         t.wf_variant = cd.wf_variant;
+        t.tail = cd.tail;
         if ((rc = oxpt_ctx_set_tuning(c, &t))) break;
         // One warm trial (first-launch costs, caches), then the fastest of
         // two timed ones.

@@ -128,10 +128,11 @@ class COptions(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block", u32), ("waves_per_cu", u32), ("max_paths", u32), ("kernel", u32), ("wf_variant", u32),
-                ("stack_lds", u32), ("bvh", u32), ("reserved", u32)]
+                ("stack_lds", u32), ("bvh", u32), ("tail", u32)]
 
 
 STACK_PRIVATE = 0xFFFF  # oxpt_tuning.stack_lds: whole traversal stack in private memory
+TAIL_NONE = 0xFFFF  # oxpt_tuning.tail: persistent kernel traces every pixel as a whole run
 
 
 class BatchItem(C.Structure):
@@ -583,8 +584,9 @@ class GpuPathTracer:
         """Name of the trace kernel the last refine launched."""
         return lib.oxpt_ctx_last_kernel(self.ctx).decode()
 
-    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh)
+    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0,
+                   tail=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail)
         rc = lib.oxpt_ctx_set_tuning(self.ctx, C.byref(t))
         if rc:
             _raise(rc, lib.oxpt_last_error(self.ctx).decode())
@@ -682,8 +684,9 @@ class Device:
         lib.oxpt_ctx_timing(self.h, 1 if reset else 0, C.byref(n), C.byref(ms))
         return n.value, ms.value
 
-    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh)
+    def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0,
+                   tail=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail)
         self._check(lib.oxpt_ctx_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def get_tuning(self) -> Tuning:

@@ -206,7 +206,28 @@ def test_kernel_forms_and_stack_split_bitwise(gpu):
     gpu.set_tuning(kernel=0)
 
 
+def test_tail_shares_bitwise(gpu):
+    """The persistent kernel's path-granular tail (the last tail/16 of the
+    frame's tiles traced one path per claim, reduced in sample order) makes
+    the same additions as whole-pixel runs: identical accumulators for every
+    tail share, on frames with ragged edge tiles, across launches split by
+    max_paths, and with punctual lights (shadow rays traced before claims)."""
+    gpu.set_environment(None)
+    for scene, (w, h), spp in ((small_terrain(), (100, 70), 6), (ptscenes.materials_scene(), (72, 40), 5)):
+        opts = oxpt.RenderOptions(width=w, height=h, samples_per_pixel=spp, max_bounces=8)
+        gpu.set_tuning(kernel=3, tail=oxpt.TAIL_NONE, max_paths=1 << 30)
+        ref = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+        # max_paths 0 keeps the previous cap: the split launches come last.
+        for tail, max_paths in ((1, 0), (8, 0), (16, 0), (0, 0), (8, w * h * 2), (16, w * h * 3)):
+            gpu.set_tuning(kernel=3, tail=tail, max_paths=max_paths)
+            got = gpu.render_hdr(scene, opts).view(np.uint32)
+            assert np.array_equal(got, ref), f"tail {tail} max_paths {max_paths}"
+    gpu.set_tuning(kernel=0, max_paths=1 << 30)
+
+
 def test_tuning_rejects_bad_stack_split(gpu):
+    with pytest.raises(oxpt.Error):
+        gpu.set_tuning(tail=17)
     with pytest.raises(oxpt.Error):
         gpu.set_tuning(stack_lds=65)
     with pytest.raises(oxpt.Error):
