@@ -142,7 +142,8 @@ typedef struct oxpt_tuning {
     uint32_t max_paths;      /* paths per launch cap (samples are split, order kept) */
     uint32_t kernel;         /* 0 = default: oxpt_refine runs the persistent megakernel at 4 waves per SIMD
                                 (frames with fewer pixels than resident lanes: the one-job batched kernel),
-                                oxpt_refine_batch the batched (one path per work item) megakernel at 3;
+                                oxpt_refine_batch the batched (one path per work item) megakernel at 4
+                                (3 for rounds of fewer than 2^24 paths);
                                 1 = plain one-pixel-per-thread, 2..6 = persistent built for 3, 4, 5, 6, 8
                                 waves per SIMD, 7 = wavefront (separate shade / extend / occlude kernels),
                                 8 = persistent built for 2 waves per SIMD */

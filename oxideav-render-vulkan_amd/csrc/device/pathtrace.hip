@@ -828,7 +828,7 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
     return hipGetLastError();
 }
 
-// Batched forms: as above, except that the default is the 3-waves build.
+// Batched forms: the same numbering (default: the 4-waves build).
 hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid,
                               uint32_t lds_depth, hipStream_t s) {
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, s, args.counters, args.n_jobs, args.max_jobs);
@@ -836,12 +836,12 @@ hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t b
     if (e != hipSuccess) return e;
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
     switch (kernel) {
-        case 3: hipLaunchKernelGGL(trace_batch<4>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        case 2: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
         case 4: hipLaunchKernelGGL(trace_batch<5>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
         case 5: hipLaunchKernelGGL(trace_batch<6>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
         case 6: hipLaunchKernelGGL(trace_batch<8>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
         case 8: hipLaunchKernelGGL(trace_batch<2>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
-        default: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        default: hipLaunchKernelGGL(trace_batch<4>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -25,12 +25,13 @@ constexpr uint32_t kCdfLds = 4352;
 // most tail samples one launch buffers (16 B each: 4 GiB).
 constexpr uint32_t kDefaultTail = 8;
 constexpr uint64_t kTailSlots = 1ull << 28;
+// Batched rounds of fewer paths than this run the 3-waves form by default.
+constexpr double kSmallRound = 16.0 * 1024 * 1024;
 
 // Waves per SIMD each megakernel form is built for (oxpt_tuning.kernel;
-// kernel 0 is 4 waves for a single refine and 3 for the batched megakernel).
-uint32_t form_waves(uint32_t kernel, bool batched) {
+// kernel 0 is 4 waves for both the single refine and the batched megakernel).
+uint32_t form_waves(uint32_t kernel) {
     switch (kernel) {
-        case 0: return batched ? 3 : 4;
         case 2: return 3;
         case 3: return 4;
         case 4: return 5;
@@ -46,7 +47,7 @@ uint32_t form_waves(uint32_t kernel, bool batched) {
 uint32_t stack_lds_entries(const oxpt_tuning &t, uint32_t block, bool batched) {
     if (t.stack_lds == OXPT_STACK_PRIVATE) return 0;
     if (t.stack_lds != 0) return t.stack_lds;
-    const uint32_t waves_cu = 4u * form_waves(t.kernel, batched);
+    const uint32_t waves_cu = 4u * form_waves(t.kernel);
     const uint32_t wg_cu = std::max(1u, waves_cu / (block / 64u));
     const uint32_t per_wg = 160u * 1024u / wg_cu;
     const uint32_t fixed = kTableLds + (batched ? 0u : kCdfLds);
@@ -645,7 +646,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     // leave most lanes idle with one pixel per lane: the default form then
     // runs as a one-job batch, whose work item is a single path (each
     // pixel's samples are still added in order, so the image is the same).
-    const uint64_t resident_lanes = (uint64_t)c->n_cu * 4u * form_waves(0, false) * 64u;
+    const uint64_t resident_lanes = (uint64_t)c->n_cu * 4u * form_waves(0) * 64u;
     if (c->tuning.kernel == 0 && (uint64_t)a->w * a->h < resident_lanes && count > 1) {
         oxpt_batch_item it;
         it.scene = s;
@@ -816,7 +817,6 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
     uint32_t spd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_count, c->tuning.max_paths / std::max<uint64_t>(total, 1)));
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
-    const uint32_t lds = stack_lds_entries(c->tuning, block, true);
     // One sample slot per traced path of a round (8x8 tiles incl. ragged
     // edges x samples of the round), sized for the largest round.
     uint64_t slots = 0;
@@ -864,6 +864,14 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
             sample_base += sorted[k].n_items * sorted[k].count;
         }
         uint32_t nj = (uint32_t)sorted.size();
+        // Default form: 4 waves per SIMD, but 3 for a small round (a few
+        // paths per resident lane), where the end of the round — one path's
+        // latency — weighs more than throughput.
+        double round_paths = 0.0;
+        for (double wk : work) round_paths += wk;
+        oxpt_tuning form = c->tuning;
+        if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = 2;
+        const uint32_t lds = stack_lds_entries(form, block, true);
         hipError_t e = hipMemcpyAsync(c->d_jobs, sorted.data(), sorted.size() * sizeof(oxd::Job), hipMemcpyHostToDevice,
                                       c->stream);
         if (e == hipSuccess)
@@ -875,7 +883,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
         if (c->use_graph) {
-            if (c->graph && (c->graph_block != block || c->graph_grid != grid || c->graph_kernel != c->tuning.kernel ||
+            if (c->graph && (c->graph_block != block || c->graph_grid != grid || c->graph_kernel != form.kernel ||
                              c->graph_lds != lds)) {
                 (void)hipGraphExecDestroy(c->graph);
                 c->graph = nullptr;
@@ -886,7 +894,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
                 if (e == hipSuccess) {
-                    hipError_t le = oxd::launch_trace_batch(args, c->tuning.kernel, block, grid, lds, c->stream);
+                    hipError_t le = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, c->stream);
                     e = hipStreamEndCapture(c->stream, &g);
                     if (e == hipSuccess) e = le;
                 }
@@ -895,13 +903,13 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 if (e != hipSuccess) return hip_fail(c, "batch graph capture", e);
                 c->graph_block = block;
                 c->graph_grid = grid;
-                c->graph_kernel = c->tuning.kernel;
+                c->graph_kernel = form.kernel;
                 c->graph_lds = lds;
             }
             e = hipGraphLaunch(c->graph, c->stream);
             c->graph_launches++;
         } else {
-            e = oxd::launch_trace_batch(args, c->tuning.kernel, block, grid, lds, c->stream);
+            e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, c->stream);
         }
         if (e != hipSuccess) return hip_fail(c, "batch launch", e);
         if (timed) {
