@@ -87,6 +87,58 @@ void oxpt_scene_destroy(oxpt_scene *scene);
 /* Device bytes held by a scene. */
 uint64_t oxpt_scene_bytes(const oxpt_scene *scene);
 
+/* Triangle soup for the device build, one record per triangle in input
+ * order (the prepared scene's triangles that GpuPathTracer::upload walks,
+ * src/pathtrace.rs:971-1118), plus the table and atlas words the caller
+ * packed (tables with OXPT_NONE in the emissive records' slot words). An
+ * attribute array may be NULL when no triangle's flags select it. */
+typedef struct oxpt_soup_desc {
+    size_t n_tris;                /* 1 .. 2^24 */
+    const float *pos;             /* 9 floats per triangle */
+    const float *normals;         /* 9, for OXPT_F_NORMALS */
+    const float *tangents;        /* 12, for OXPT_F_TANGENTS */
+    const float *uv0;             /* 6, for OXPT_F_UV0 */
+    const float *uv1;             /* 6, for OXPT_F_UV1 */
+    const float *colors;          /* 12, for OXPT_F_COLORS */
+    const uint32_t *material;     /* material index */
+    const uint8_t *flags;         /* OXPT_F_* */
+    const uint32_t *light_of;     /* emissive-light index or OXPT_NONE; NULL: none */
+    uint32_t n_emissive;
+    uint32_t elight_offset;       /* table word of emissive record 0 */
+    const uint32_t *tables;
+    size_t table_words;
+    const uint16_t *atlas;        /* binary16 RGBA texels, may be NULL */
+    uint32_t atlas_side;
+    uint32_t atlas_layers;
+} oxpt_soup_desc;
+
+/* What the device build produced: the Params words that depend on the
+ * geometry (node count: env[3], triangle base: offs2[2], 4-wide root:
+ * offs2[3]), the tree depths and timings. */
+typedef struct oxpt_build_info {
+    uint32_t node_count, tri_base, wide_base, bvh_depth, wide_depth;
+    uint32_t pad;
+    double upload_ms;  /* soup, tables and atlas host -> device */
+    double build_ms;   /* BVH, triangle / attribute gather, 4-wide collapse */
+} oxpt_build_info;
+
+/* Builds the scene on the device: the binned-SAH BVH, the triangles and
+ * attributes in leaf-slot order and the 4-wide nodes, the same words the
+ * host packer writes (the tree depends only on the triangle set: leaves
+ * list ascending triangle ids). Fills the emissive records' slot words of
+ * the uploaded tables; emissive_slots (n_emissive words, may be NULL)
+ * receives them for the caller's own table copy. Replaces the host BVH
+ * and geometry packing of GpuPathTracer::upload (src/pathtrace.rs:970-1118);
+ * the reference takes its tree from oxideav-mesh3d's Bvh on the CPU. */
+int oxpt_scene_build(oxpt_ctx *ctx, const oxpt_soup_desc *desc, uint32_t *emissive_slots, oxpt_scene **out,
+                     oxpt_build_info *info);
+/* Copy a resident scene's words to host memory, blocking: which = 0 geom
+ * (float4 words), 1 attrs (float4 words), 2 tables (u32), 3 atlas (u16);
+ * *words (may be NULL) receives the element count, host_out may be NULL
+ * to ask for it. No counterpart in the reference (tests and tools). */
+int oxpt_scene_read(oxpt_ctx *ctx, const oxpt_scene *scene, int which, void *host_out, size_t bytes,
+                    size_t *words);
+
 /* Accumulator of w*h float4 (src/pathtrace.rs:551-561), zeroed. */
 int oxpt_accum_create(oxpt_ctx *ctx, uint32_t w, uint32_t h, oxpt_accum **out);
 void oxpt_accum_destroy(oxpt_accum *acc);
@@ -342,6 +394,15 @@ int oxpt_tracer_params(oxpt_tracer *t, oxpt_params *out);
  * environment included), 3 atlas (u16 count). */
 int oxpt_tracer_words(oxpt_tracer *t, int which, const void **ptr, size_t *count);
 int oxpt_tracer_atlas_shape(oxpt_tracer *t, uint32_t *side, uint32_t *layers);
+/* Prepare scenes on the device (oxpt_scene_build: BVH, triangle and
+ * attribute words, 4-wide nodes) instead of on the host; the words and
+ * images are the same. Takes effect at the next sync (the scene is
+ * re-prepared). No counterpart in the reference, whose upload packs on
+ * the CPU (src/pathtrace.rs:970-1239). */
+int oxpt_tracer_set_device_build(oxpt_tracer *t, int enable);
+/* Node counts and timings of the last device-prepared scene (zeros after
+ * a host-packed one). */
+int oxpt_tracer_build_info(const oxpt_tracer *t, oxpt_build_info *out);
 
 /* Host-only packing (no device): the words and Params a sync + refine would
  * hand the kernels (upload + ensure_binding, src/pathtrace.rs:847-1239).

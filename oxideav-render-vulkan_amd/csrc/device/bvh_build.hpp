@@ -1,0 +1,59 @@
+// Device scene preparation: the binned-SAH BVH, the leaf-ordered triangle
+// and attribute words and the 4-wide nodes, built on the GPU from the
+// triangle soup. The words are the host packer's (csrc/host/bvh.cpp,
+// csrc/host/pack.cpp) bit for bit; the reference does this on the host in
+// GpuPathTracer::upload (src/pathtrace.rs:970-1239).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <functional>
+#include <string>
+
+namespace oxd {
+
+// The soup in device memory, one record per triangle in input order. An
+// attribute array may be null when no triangle's flags select it.
+struct SoupDev {
+    uint32_t n = 0;
+    const float *pos = nullptr;       // 9 per triangle
+    const float *normals = nullptr;   // 9
+    const float *tangents = nullptr;  // 12
+    const float *uv0 = nullptr;       // 6
+    const float *uv1 = nullptr;       // 6
+    const float *colors = nullptr;    // 12
+    const uint32_t *material = nullptr;
+    const uint8_t *flags = nullptr;   // OXPT_F_* per triangle
+    const uint32_t *light_of = nullptr;  // emissive-light index or OXPT_NONE
+};
+
+// Device allocation through the caller (the context's budget accounting).
+struct DevAlloc {
+    std::function<int(size_t, void **, const char *)> alloc;
+    std::function<void(void *, size_t)> release;
+};
+
+struct BuiltGeometry {
+    float4 *geom = nullptr;  // binary nodes | triangles | pad | 4-wide nodes
+    size_t geom_vec4 = 0;
+    float4 *attrs = nullptr;
+    size_t attrs_vec4 = 0;
+    uint32_t node_count = 0, tri_base = 0, wide_base = 0;
+    uint32_t bvh_depth = 0, wide_depth = 0;
+    double ms = 0.0;  // wall time of the build, upload of the soup excluded
+};
+
+// Subtrees of at most this many triangles are finished by one wave in LDS.
+constexpr uint32_t kBuildSmall = 512;
+// Largest soup the device builder takes (its scans cover 2^24 slots).
+constexpr uint32_t kBuildMaxTris = 1u << 24;
+
+// Builds everything on stream `s`. `emissive_slots` (n_emissive words,
+// device memory, may be null) receives the leaf slot of every emissive
+// triangle, indexed by its light index. Returns 0 or a negative OXPT_E_*
+// code with `err` set; temporaries are released either way.
+int build_geometry(const SoupDev &soup, uint32_t n_emissive, uint32_t *emissive_slots, hipStream_t s,
+                   const DevAlloc &a, BuiltGeometry *out, std::string &err);
+
+}  // namespace oxd

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "oxpt/oxpt.h"
+#include "bvh_build.hpp"
 #include "pt_kernels.hpp"
 #include "wavefront.hpp"
 
@@ -117,6 +119,8 @@ struct oxpt_scene {
     uint32_t atlas_side = 1, atlas_layers = 1;
     size_t table_words = 0;
     size_t geom_vec4 = 0;
+    size_t attrs_vec4 = 0;
+    size_t atlas_words = 0;  // u16 texels held (4 when the scene has no atlas)
     size_t bytes = 0;
 };
 
@@ -476,10 +480,12 @@ int oxpt_scene_upload(oxpt_ctx *c, const oxpt_scene_desc *d, oxpt_scene **out) {
     s->ctx = c;
     s->table_words = d->table_words;
     s->geom_vec4 = d->geom_vec4;
+    s->attrs_vec4 = d->attrs_vec4;
     s->atlas_side = std::max(d->atlas_side, 1u);
     s->atlas_layers = std::max(d->atlas_layers, 1u);
     size_t gb = d->geom_vec4 * 16, ab = d->attrs_vec4 * 16, tb = d->table_words * 4;
     size_t xb = d->atlas ? (size_t)s->atlas_side * s->atlas_side * s->atlas_layers * 8 : 8;
+    s->atlas_words = xb / 2;
     void *p;
     int rc;
     if ((rc = dalloc(c, gb, &p, "BVH + triangle buffer"))) goto err;
@@ -519,6 +525,165 @@ int oxpt_scene_upload(oxpt_ctx *c, const oxpt_scene_desc *d, oxpt_scene **out) {
 err:
     oxpt_scene_destroy(s);
     return rc;
+}
+
+int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_slots, oxpt_scene **out,
+                     oxpt_build_info *info) {
+    if (!c || !d || !out) return OXPT_E_ARG;
+    *out = nullptr;
+    if (!d->pos || !d->material || !d->flags || !d->tables || d->table_words == 0)
+        return fail(c, OXPT_E_ARG, "scene build: positions, materials, flags and tables are required");
+    if (d->n_tris == 0 || d->n_tris > oxd::kBuildMaxTris)
+        return fail(c, OXPT_E_INVALID, "scene build: the device builder takes 1 to 16777216 triangles");
+    if (d->n_emissive > 0 && (!d->light_of || (size_t)d->elight_offset + 4ull * d->n_emissive > d->table_words))
+        return fail(c, OXPT_E_ARG, "scene build: emissive records need light_of and fit the tables");
+    const size_t n = d->n_tris;
+    uint32_t any = 0;
+    for (size_t i = 0; i < n; i++) any |= d->flags[i];
+    struct Arr {
+        const void *src;
+        size_t bytes;
+        uint32_t flag;
+        const char *what;
+    };
+    const Arr arrs[9] = {{d->pos, 36 * n, 0, "positions"},
+                         {d->normals, 36 * n, OXPT_F_NORMALS, "normals"},
+                         {d->tangents, 48 * n, OXPT_F_TANGENTS, "tangents"},
+                         {d->uv0, 24 * n, OXPT_F_UV0, "uv0"},
+                         {d->uv1, 24 * n, OXPT_F_UV1, "uv1"},
+                         {d->colors, 48 * n, OXPT_F_COLORS, "colors"},
+                         {d->material, 4 * n, 0, "materials"},
+                         {d->flags, n, 0, "flags"},
+                         {d->light_of, 4 * n, 0, "light ids"}};
+    for (const Arr &a : arrs)
+        if (a.flag && (any & a.flag) && !a.src)
+            return fail(c, OXPT_E_ARG, std::string("scene build: triangles select ") + a.what + " but none given");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    const auto t0 = std::chrono::steady_clock::now();
+    // The soup and the emissive slot words live on the device for the build only.
+    void *dev[10] = {};
+    size_t dev_bytes[10] = {};
+    auto release = [&]() {
+        (void)hipStreamSynchronize(c->stream);
+        for (int i = 0; i < 10; i++) dfree(c, dev[i], dev_bytes[i]);
+    };
+    int rc = OXPT_OK;
+    for (int i = 0; i < 9 && rc == OXPT_OK; i++) {
+        const Arr &a = arrs[i];
+        if (!a.src || (a.flag && !(any & a.flag))) continue;
+        dev_bytes[i] = a.bytes;
+        if ((rc = dalloc(c, a.bytes, &dev[i], "scene build: soup"))) break;
+        hipError_t e = hipMemcpyAsync(dev[i], a.src, a.bytes, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, "scene build: soup upload", e);
+    }
+    if (rc == OXPT_OK && d->n_emissive > 0) {
+        dev_bytes[9] = 4ull * d->n_emissive;
+        rc = dalloc(c, dev_bytes[9], &dev[9], "scene build: emissive slots");
+    }
+    if (rc == OXPT_OK) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, "scene build: soup upload", e);
+    }
+    if (rc) {
+        release();
+        return rc;
+    }
+    const double upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    oxd::SoupDev sd;
+    sd.n = (uint32_t)n;
+    sd.pos = (const float *)dev[0];
+    sd.normals = (const float *)dev[1];
+    sd.tangents = (const float *)dev[2];
+    sd.uv0 = (const float *)dev[3];
+    sd.uv1 = (const float *)dev[4];
+    sd.colors = (const float *)dev[5];
+    sd.material = (const uint32_t *)dev[6];
+    sd.flags = (const uint8_t *)dev[7];
+    sd.light_of = (const uint32_t *)dev[8];
+    oxd::DevAlloc al;
+    al.alloc = [c](size_t bytes, void **p, const char *what) { return dalloc(c, bytes, p, what); };
+    al.release = [c](void *p, size_t bytes) { dfree(c, p, bytes); };
+    oxd::BuiltGeometry bg;
+    std::string err;
+    rc = oxd::build_geometry(sd, d->n_emissive, (uint32_t *)dev[9], c->stream, al, &bg, err);
+    if (rc) {
+        release();
+        return fail(c, rc, err);
+    }
+    // Tables with the emissive slot words filled in, then the atlas.
+    std::vector<uint32_t> tables(d->tables, d->tables + d->table_words);
+    std::vector<uint32_t> slots(d->n_emissive);
+    if (d->n_emissive > 0) {
+        hipError_t e = hipMemcpy(slots.data(), dev[9], 4ull * d->n_emissive, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(c, "scene build: emissive slots", e);
+        for (uint32_t i = 0; i < d->n_emissive; i++) tables[d->elight_offset + 4ull * i + 3] = slots[i];
+    }
+    release();
+    auto *s = new oxpt_scene();
+    s->ctx = c;
+    s->geom = bg.geom;
+    s->attrs = bg.attrs;
+    s->geom_vec4 = bg.geom_vec4;
+    s->attrs_vec4 = bg.attrs_vec4;
+    s->table_words = tables.size();
+    s->atlas_side = std::max(d->atlas_side, 1u);
+    s->atlas_layers = std::max(d->atlas_layers, 1u);
+    const size_t tb = tables.size() * 4;
+    const size_t xb = d->atlas ? (size_t)s->atlas_side * s->atlas_side * s->atlas_layers * 8 : 8;
+    s->atlas_words = xb / 2;
+    s->bytes = bg.geom_vec4 * 16 + bg.attrs_vec4 * 16 + tb + xb;
+    void *p = nullptr;
+    if (rc == OXPT_OK && (rc = dalloc(c, tb, &p, "scene tables")) == OXPT_OK) s->tables = (uint32_t *)p;
+    if (rc == OXPT_OK && (rc = dalloc(c, xb, &p, "texture atlas")) == OXPT_OK) s->atlas = (uint16_t *)p;
+    if (rc == OXPT_OK) {
+        s->host_tables = tables;
+        hipError_t e = hipMemcpyAsync(s->tables, tables.data(), tb, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = d->atlas ? hipMemcpyAsync(s->atlas, d->atlas, xb, hipMemcpyHostToDevice, c->stream)
+                         : hipMemsetAsync(s->atlas, 0, xb, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, "scene build: tables / atlas upload", e);
+    }
+    if (rc) {
+        // Account the build's buffers to the scene so destroy releases them.
+        if (!s->tables) s->bytes -= tb;
+        if (!s->atlas) s->bytes -= xb;
+        oxpt_scene_destroy(s);
+        return rc;
+    }
+    if (emissive_slots && d->n_emissive > 0) std::memcpy(emissive_slots, slots.data(), 4ull * d->n_emissive);
+    if (info) {
+        info->node_count = bg.node_count;
+        info->tri_base = bg.tri_base;
+        info->wide_base = bg.wide_base;
+        info->bvh_depth = bg.bvh_depth;
+        info->wide_depth = bg.wide_depth;
+        info->pad = 0;
+        info->upload_ms = upload_ms;
+        info->build_ms = bg.ms;
+    }
+    *out = s;
+    return OXPT_OK;
+}
+
+int oxpt_scene_read(oxpt_ctx *c, const oxpt_scene *s, int which, void *host_out, size_t bytes, size_t *words) {
+    if (!c || !s || s->ctx != c) return OXPT_E_ARG;
+    const void *src;
+    size_t count, elem;
+    switch (which) {
+        case 0: src = s->geom; count = s->geom_vec4; elem = 16; break;
+        case 1: src = s->attrs; count = s->attrs_vec4; elem = 16; break;
+        case 2: src = s->tables; count = s->table_words; elem = 4; break;
+        case 3: src = s->atlas; count = s->atlas_words; elem = 2; break;
+        default: return OXPT_E_ARG;
+    }
+    if (words) *words = count;
+    if (!host_out) return OXPT_OK;
+    if (bytes < count * elem) return fail(c, OXPT_E_ARG, "scene read: buffer too small");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(host_out, src, count * elem, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "scene read", e);
 }
 
 int oxpt_scene_set_tables(oxpt_ctx *c, oxpt_scene *s, const uint32_t *tables, size_t words) {

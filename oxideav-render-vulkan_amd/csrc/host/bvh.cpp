@@ -9,6 +9,18 @@
 // only the layout is fixed by the kernel. Images do not depend on the
 // topology except through exact-tie order, which the brute-force
 // closest-hit test covers.
+//
+// The tree is a function of the triangle set alone, so the device builder
+// (csrc/device/bvh_build.hip) reproduces it word for word:
+//  - triangle boxes hold no -0 and no NaN (coordinates are turned from -0
+//    into +0, a NaN coordinate is skipped), so box unions do not depend on
+//    the order triangles are visited in;
+//  - split decisions use only per-bin minima, maxima and counts;
+//  - partitions are stable, so every node's triangles stay in ascending id
+//    order: leaves list ascending ids, and the object-median fallback (all
+//    centroids equal) splits a node at its middle position;
+//  - nodes are numbered depth first, left subtree first (each inner node's
+//    child pair is appended when the node is visited).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -27,6 +39,10 @@ constexpr uint32_t kMaxDepth = 56;    // ... unless the depth cap (stack is 64) 
 constexpr float kTravCost = 1.0f;
 constexpr float kTriCost = 1.0f;
 
+// Boxes grow by std::min / std::max, which skip a NaN operand (it compares
+// false) and are order-independent on the values they see: triangle boxes
+// are built from coordinates with -0 turned into +0 (see build_bvh), and
+// centroids of such boxes are never -0.
 struct Box {
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -49,6 +65,15 @@ struct Box {
     }
 };
 
+// Bin of centroid coordinate c in [lo, lo + kBins / scale]; NaN goes to
+// bin 0 and overflow to the last bin, the same on every compiler.
+inline int bin_of(float c, float lo, float scale) {
+    float f = (c - lo) * scale;
+    if (!(f > 0.0f)) return 0;
+    if (f >= (float)kBins) return kBins - 1;
+    return (int)f;
+}
+
 struct Node {
     Box box;
     uint32_t first = 0, count = 0;
@@ -59,10 +84,11 @@ struct Builder {
     const std::vector<float> &cen;  // 3 per tri
     std::vector<uint32_t> &ids;
     std::vector<Node> nodes;
+    std::vector<uint32_t> scratch;  // right-hand ids of the partition in progress
     uint32_t depth = 0;
 
     Builder(const std::vector<Box> &t, const std::vector<float> &c, std::vector<uint32_t> &i)
-        : tb(t), cen(c), ids(i) {}
+        : tb(t), cen(c), ids(i), scratch(i.size()) {}
 
     // Builds the subtree for ids[b, e) into `nodes[ni]`, children appended.
     void build(uint32_t ni, uint32_t b, uint32_t e, uint32_t d) {
@@ -90,8 +116,7 @@ struct Builder {
             uint32_t cnt[kBins] = {0};
             for (uint32_t i = b; i < e; i++) {
                 uint32_t t = ids[i];
-                int bi = (int)((cen[3 * (size_t)t + ax] - cb.lo[ax]) * scale);
-                bi = std::min(std::max(bi, 0), kBins - 1);
+                int bi = bin_of(cen[3 * (size_t)t + ax], cb.lo[ax], scale);
                 bins[bi].grow(tb[t]);
                 cnt[bi]++;
             }
@@ -129,39 +154,29 @@ struct Builder {
         }
         uint32_t m;
         if (best_axis >= 0) {
+            // Both sides are non-empty: the winning split has triangles on each.
             int ax = best_axis;
             float scale = (float)kBins / (cb.hi[ax] - cb.lo[ax]);
             float lo = cb.lo[ax];
-            auto mid = std::partition(ids.begin() + b, ids.begin() + e, [&](uint32_t t) {
-                int bi = (int)((cen[3 * (size_t)t + ax] - lo) * scale);
-                bi = std::min(std::max(bi, 0), kBins - 1);
-                return bi < best_split;
-            });
-            m = (uint32_t)(mid - ids.begin());
+            // Stable partition through the scratch buffer: left ids in
+            // place, right ids after them, both in their original order.
+            uint32_t l = b, r = 0;
+            for (uint32_t i = b; i < e; i++) {
+                uint32_t t = ids[i];
+                if (bin_of(cen[3 * (size_t)t + ax], lo, scale) < best_split) ids[l++] = t;
+                else scratch[r++] = t;
+            }
+            std::copy(scratch.begin(), scratch.begin() + r, ids.begin() + l);
+            m = l;
         } else {
-            m = b;  // all centroids coincide
-        }
-        if (m == b || m == e) {
+            // All centroids coincide: leaf, or object median. The ids are in
+            // ascending order, so the median of (centroid, id) is the middle.
             if (n <= kHardLeaf) {
                 nodes[ni].first = b;
                 nodes[ni].count = n;
                 return;
             }
-            // Object median along the widest centroid (or box) axis.
-            int ax = 0;
-            float w = -1.0f;
-            for (int k = 0; k < 3; k++) {
-                float ext = cb.hi[k] - cb.lo[k];
-                if (ext > w) {
-                    w = ext;
-                    ax = k;
-                }
-            }
             m = b + n / 2;
-            std::nth_element(ids.begin() + b, ids.begin() + m, ids.begin() + e, [&](uint32_t x, uint32_t y) {
-                float cx = cen[3 * (size_t)x + ax], cy = cen[3 * (size_t)y + ax];
-                return cx < cy || (cx == cy && x < y);
-            });
         }
         uint32_t c = (uint32_t)nodes.size();
         nodes.emplace_back();
@@ -190,11 +205,12 @@ uint32_t build_bvh(const Soup &soup, std::vector<uint32_t> &node_words, std::vec
     std::vector<float> cen(3 * n);
     auto prep = [&](size_t b, size_t e) {
         for (size_t t = b; t < e; t++) {
-            const float *p = &soup.pos[9 * t];
             Box bx;
-            bx.grow(p);
-            bx.grow(p + 3);
-            bx.grow(p + 6);
+            for (int v = 0; v < 3; v++) {
+                float q[3];
+                for (int k = 0; k < 3; k++) q[k] = soup.pos[9 * t + 3 * v + k] + 0.0f;  // -0 -> +0
+                bx.grow(q);
+            }
             tb[t] = bx;
             for (int k = 0; k < 3; k++) cen[3 * t + k] = 0.5f * (bx.lo[k] + bx.hi[k]);
         }

@@ -308,7 +308,7 @@ CameraFrame resolve_camera(const CameraSource &s, uint32_t w, uint32_t h) {
     return f;
 }
 
-Packed pack_scene(const HostScene &s) {
+Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *light_of_out) {
     Packed pk;
     const Soup &soup = s.soup;
     size_t ntri = soup.count();
@@ -346,9 +346,12 @@ Packed pack_scene(const HostScene &s) {
         pmf[i] = cdf[i] - prev;
     }
 
-    // ---- BVH + triangles in leaf-slot order.
+    if (light_of_out) *light_of_out = light_of;
+    // ---- BVH + triangles in leaf-slot order (or left to the device
+    // builder: then the emissive records' slot words are OXPT_NONE until
+    // the caller fills them in).
     std::vector<uint32_t> node_words, order;
-    pk.bvh_depth = build_bvh(soup, node_words, order);
+    if (geometry) pk.bvh_depth = build_bvh(soup, node_words, order);
     pk.node_count = (uint32_t)(node_words.size() / 8);
     pk.tri_base = pk.node_count * 2;
     std::vector<uint32_t> slot_of(ntri, OXPT_NONE);
@@ -402,7 +405,7 @@ Packed pack_scene(const HostScene &s) {
         fill(0, order.size());
     }
     // ---- 4-wide nodes after the triangles, 128-byte aligned.
-    if (!order.empty()) {
+    if (!order.empty() && geometry) {
         std::vector<float> wide;
         pk.wide_depth = build_wide(node_words, wide);
         if (!wide.empty()) {
@@ -412,8 +415,10 @@ Packed pack_scene(const HostScene &s) {
             pk.geom.insert(pk.geom.end(), wide.begin(), wide.end());
         }
     }
-    if (pk.geom.empty()) pk.geom.resize(8, 0.0f);
-    if (pk.attrs.empty()) pk.attrs.resize(48, 0.0f);
+    if (geometry) {
+        if (pk.geom.empty()) pk.geom.resize(8, 0.0f);
+        if (pk.attrs.empty()) pk.attrs.resize(48, 0.0f);
+    }
 
     // ---- Tables.
     std::vector<uint32_t> &t = pk.tables;
@@ -531,7 +536,7 @@ Packed pack_scene(const HostScene &s) {
     pk.off_elights = (uint32_t)t.size();
     for (size_t li = 0; li < elight_tris.size(); li++) {
         uint32_t g = elight_tris[li];
-        t.insert(t.end(), {fb(cdf[li]), fb(pmf[li]), g, slot_of[g]});
+        t.insert(t.end(), {fb(cdf[li]), fb(pmf[li]), g, geometry ? slot_of[g] : OXPT_NONE});
     }
     pk.n_punctual = (uint32_t)s.lights.size();
     pk.n_emissive = (uint32_t)elight_tris.size();

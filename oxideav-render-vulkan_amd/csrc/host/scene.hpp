@@ -99,8 +99,12 @@ constexpr uint32_t kWideStack = 96;
 // 3 * depth <= kWideStack); the kernels then walk the binary tree.
 uint32_t build_wide(const std::vector<uint32_t> &node_words, std::vector<float> &wide);
 
-// Pack everything but the environment (GpuPathTracer::upload).
-Packed pack_scene(const HostScene &s);
+// Pack everything but the environment (GpuPathTracer::upload). With
+// geometry = false the BVH, triangle and attribute words are left to the
+// device builder (oxpt_scene_build): geom / attrs stay empty and the
+// emissive records' slot words hold OXPT_NONE. light_of_out receives each
+// triangle's emissive-light index (OXPT_NONE for none).
+Packed pack_scene(const HostScene &s, bool geometry = true, std::vector<uint32_t> *light_of_out = nullptr);
 // Environment section (env_words, src/pathtrace.rs:1428-1461).
 EnvWords env_words(const EnvMap &e);
 

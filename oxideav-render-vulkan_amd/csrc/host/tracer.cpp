@@ -62,7 +62,71 @@ struct oxpt_tracer {
         samples = 0;
         if (accum) oxpt_accum_clear(ctx, accum);
     }
+    // Device scene preparation (oxpt_scene_build) instead of host BVH and
+    // geometry packing; the words are the same either way.
+    bool device_build = false;
+    bool words_on_device = false;  // geom / attrs of `packed` not downloaded yet
+    oxpt_build_info build_info{};
+
+    int upload_device(const oxpt::HostScene &s) {
+        std::vector<uint32_t> light_of;
+        auto pk = std::make_unique<oxpt::Packed>(oxpt::pack_scene(s, false, &light_of));
+        if (dscene) {
+            oxpt_scene_destroy(dscene);
+            dscene = nullptr;
+        }
+        const oxpt::Soup &so = s.soup;
+        oxpt_soup_desc d{};
+        d.n_tris = so.count();
+        d.pos = so.pos.data();
+        d.normals = so.normals.empty() ? nullptr : so.normals.data();
+        d.tangents = so.tangents.empty() ? nullptr : so.tangents.data();
+        d.uv0 = so.uv0.empty() ? nullptr : so.uv0.data();
+        d.uv1 = so.uv1.empty() ? nullptr : so.uv1.data();
+        d.colors = so.colors.empty() ? nullptr : so.colors.data();
+        d.material = so.material.data();
+        d.flags = so.flags.data();
+        d.light_of = light_of.data();
+        d.n_emissive = pk->n_emissive;
+        d.elight_offset = pk->off_elights;
+        d.tables = pk->tables.data();
+        d.table_words = pk->tables.size();
+        d.atlas = pk->atlas.empty() ? nullptr : pk->atlas.data();
+        d.atlas_side = pk->atlas_side;
+        d.atlas_layers = pk->atlas_layers;
+        std::vector<uint32_t> slots(pk->n_emissive);
+        int rc = oxpt_scene_build(ctx, &d, slots.data(), &dscene, &build_info);
+        if (rc) return dev_fail(rc);
+        for (uint32_t i = 0; i < pk->n_emissive; i++) pk->tables[pk->off_elights + 4 * i + 3] = slots[i];
+        pk->node_count = build_info.node_count;
+        pk->tri_base = build_info.tri_base;
+        pk->wide_base = build_info.wide_base;
+        pk->bvh_depth = build_info.bvh_depth;
+        pk->wide_depth = build_info.wide_depth;
+        packed = std::move(pk);
+        words_on_device = true;
+        cam_src = oxpt::camera_source(s);
+        return OXPT_OK;
+    }
+    // Host copies of the device-built geometry words, on request.
+    int fetch_words() {
+        if (!words_on_device) return OXPT_OK;
+        size_t gv = 0, av = 0;
+        int rc = oxpt_scene_read(ctx, dscene, 0, nullptr, 0, &gv);
+        if (!rc) rc = oxpt_scene_read(ctx, dscene, 1, nullptr, 0, &av);
+        if (rc) return dev_fail(rc);
+        packed->geom.resize(gv * 4);
+        packed->attrs.resize(av * 4);
+        rc = oxpt_scene_read(ctx, dscene, 0, packed->geom.data(), gv * 16, nullptr);
+        if (!rc) rc = oxpt_scene_read(ctx, dscene, 1, packed->attrs.data(), av * 16, nullptr);
+        if (rc) return dev_fail(rc);
+        words_on_device = false;
+        return OXPT_OK;
+    }
     int upload(const oxpt::HostScene &s) {
+        build_info = oxpt_build_info{};
+        if (device_build && s.soup.count() > 0) return upload_device(s);
+        words_on_device = false;
         auto pk = std::make_unique<oxpt::Packed>(oxpt::pack_scene(s));
         if (dscene) {
             oxpt_scene_destroy(dscene);
@@ -358,6 +422,8 @@ int oxpt_tracer_words(oxpt_tracer *t, int which, const void **ptr, size_t *count
     if (!t || !ptr || !count) return OXPT_E_ARG;
     if (!t->packed) return t->fail(OXPT_E_ARG, "words: not synced");
     if (int rc = t->ensure_binding()) return rc;
+    if (which <= 1)
+        if (int rc = t->fetch_words()) return rc;
     switch (which) {
         case 0: *ptr = t->packed->geom.data(); *count = t->packed->geom.size() / 4; return OXPT_OK;
         case 1: *ptr = t->packed->attrs.data(); *count = t->packed->attrs.size() / 4; return OXPT_OK;
@@ -365,6 +431,17 @@ int oxpt_tracer_words(oxpt_tracer *t, int which, const void **ptr, size_t *count
         case 3: *ptr = t->packed->atlas.data(); *count = t->packed->atlas.size(); return OXPT_OK;
         default: return OXPT_E_ARG;
     }
+}
+int oxpt_tracer_set_device_build(oxpt_tracer *t, int enable) {
+    if (!t) return OXPT_E_ARG;
+    if (t->device_build != (enable != 0)) t->dirty = true;  // re-prepare on the next sync
+    t->device_build = enable != 0;
+    return OXPT_OK;
+}
+int oxpt_tracer_build_info(const oxpt_tracer *t, oxpt_build_info *out) {
+    if (!t || !out) return OXPT_E_ARG;
+    *out = t->build_info;
+    return OXPT_OK;
 }
 int oxpt_tracer_atlas_shape(oxpt_tracer *t, uint32_t *side, uint32_t *layers) {
     if (!t || !t->packed) return OXPT_E_ARG;

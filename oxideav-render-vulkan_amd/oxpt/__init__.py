@@ -82,6 +82,20 @@ class SceneDesc(C.Structure):
     ]
 
 
+class SoupDesc(C.Structure):
+    _fields_ = [
+        ("n_tris", sz), ("pos", P(f32)), ("normals", P(f32)), ("tangents", P(f32)), ("uv0", P(f32)),
+        ("uv1", P(f32)), ("colors", P(f32)), ("material", P(u32)), ("flags", P(C.c_uint8)),
+        ("light_of", P(u32)), ("n_emissive", u32), ("elight_offset", u32), ("tables", P(u32)),
+        ("table_words", sz), ("atlas", P(C.c_uint16)), ("atlas_side", u32), ("atlas_layers", u32),
+    ]
+
+
+class BuildInfo(C.Structure):
+    _fields_ = [("node_count", u32), ("tri_base", u32), ("wide_base", u32), ("bvh_depth", u32),
+                ("wide_depth", u32), ("pad", u32), ("upload_ms", C.c_double), ("build_ms", C.c_double)]
+
+
 class TexBinding(C.Structure):
     _fields_ = [("texture", i32), ("uv_set", u32), ("xform", f32 * 6)]
 
@@ -179,6 +193,8 @@ _sig("oxpt_ctx_bytes_in_use", u64, _vp)
 _sig("oxpt_scene_upload", C.c_int, _vp, P(SceneDesc), P(_vp))
 _sig("oxpt_scene_set_tables", C.c_int, _vp, _vp, P(u32), sz)
 _sig("oxpt_scene_destroy", None, _vp)
+_sig("oxpt_scene_build", C.c_int, _vp, P(SoupDesc), P(u32), P(_vp), P(BuildInfo))
+_sig("oxpt_scene_read", C.c_int, _vp, _vp, C.c_int, _vp, sz, P(sz))
 _sig("oxpt_scene_bytes", u64, _vp)
 _sig("oxpt_accum_create", C.c_int, _vp, u32, u32, P(_vp))
 _sig("oxpt_accum_destroy", None, _vp)
@@ -238,6 +254,8 @@ _sig("oxpt_tracer_device_scene", _vp, _vp)
 _sig("oxpt_tracer_params", C.c_int, _vp, P(Params))
 _sig("oxpt_tracer_words", C.c_int, _vp, C.c_int, P(_vp), P(sz))
 _sig("oxpt_tracer_atlas_shape", C.c_int, _vp, P(u32), P(u32))
+_sig("oxpt_tracer_set_device_build", C.c_int, _vp, C.c_int)
+_sig("oxpt_tracer_build_info", C.c_int, _vp, P(BuildInfo))
 _sig("oxpt_pack", C.c_int, _vp, _vp, P(COptions), P(_vp))
 _sig("oxpt_packed_free", None, _vp)
 _sig("oxpt_packed_words", C.c_int, _vp, C.c_int, P(_vp), P(sz))
@@ -574,6 +592,30 @@ class GpuPathTracer:
         s, l = u32(), u32()
         self._check(lib.oxpt_tracer_atlas_shape(self.h, C.byref(s), C.byref(l)), "atlas_shape")
         return s.value, l.value
+
+    def set_device_build(self, enable: bool = True) -> None:
+        """Prepare scenes on the device (BVH, triangle / attribute words,
+        4-wide nodes) from the next sync on; the words are the host packer's."""
+        self._check(lib.oxpt_tracer_set_device_build(self.h, 1 if enable else 0), "set_device_build")
+
+    def build_info(self) -> BuildInfo:
+        b = BuildInfo()
+        self._check(lib.oxpt_tracer_build_info(self.h, C.byref(b)), "build_info")
+        return b
+
+    def scene_words(self, which: int) -> np.ndarray:
+        """The resident scene's words as the device holds them (oxpt_scene_read)."""
+        scene = self.device_scene()
+        n = sz()
+        rc = lib.oxpt_scene_read(self.ctx, scene, which, None, 0, C.byref(n))
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        dt, per, elem = [(np.float32, 4, 16), (np.float32, 4, 16), (np.uint32, 1, 4), (np.uint16, 1, 2)][which]
+        out = np.zeros(n.value * per, dtype=dt)
+        rc = lib.oxpt_scene_read(self.ctx, scene, which, out.ctypes.data_as(_vp), n.value * elem, None)
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        return out
 
     def timing(self, reset: bool = False) -> Tuple[int, float]:
         n, ms = u64(), C.c_double()

@@ -195,6 +195,25 @@ def test_bvh_leaf_slots_cover_every_triangle_once():
     gids = tris[:, 3].view(np.uint32)
     assert np.array_equal(np.sort(gids), np.arange(scene.triangle_count(), dtype=np.uint32))
     assert pk.bvh_depth < 64
+    # Stable partitions: every leaf lists its triangles in ascending id order
+    # (the canonical tree the device builder reproduces), and every node's
+    # box is exactly the union of its triangles' boxes.
+    tri_lo = np.minimum(np.minimum(tris[:, 0:3], tris[:, 4:7]), tris[:, 8:11]) + np.float32(0.0)
+    tri_hi = np.maximum(np.maximum(tris[:, 0:3], tris[:, 4:7]), tris[:, 8:11]) + np.float32(0.0)
+
+    def span(i):
+        if count[i] > 0:
+            return first[i], first[i] + count[i]
+        return span(first[i])[0], span(first[i] + 1)[1]
+
+    for i in range(n_nodes):
+        if i == 1:
+            continue
+        b, e = span(i)
+        if count[i] > 0:
+            assert np.all(np.diff(gids[b:e].astype(np.int64)) > 0), f"leaf {i} ids not ascending"
+        assert np.array_equal(nodes[i, :3], tri_lo[b:e].min(axis=0)), f"node {i} lo"
+        assert np.array_equal(nodes[i, 4:7], tri_hi[b:e].max(axis=0)), f"node {i} hi"
 
 
 @pytest.mark.parametrize("gen", ["terrain", "room", "coincident"])
