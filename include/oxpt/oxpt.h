@@ -87,6 +87,32 @@ void oxpt_scene_destroy(oxpt_scene *scene);
 /* Device bytes held by a scene. */
 uint64_t oxpt_scene_bytes(const oxpt_scene *scene);
 
+/* Texture atlas built on the device: each chain decodes one texture's
+ * RGBA8 level 0 (colour channels through decode_lut[0..255] when srgb,
+ * else decode_lut[256..511]; alpha through decode_lut[256..511]), box-
+ * filters its mips (2x2, edge texels repeated) and writes every kept level
+ * as binary16 RGBA (round to nearest even) at the rectangles its texture
+ * descriptor in the tables lists ([levels, flags, w, h, skip, kept, 0, 0]
+ * + kept x {layer, x, y, w | h << 16}). Replaces AtlasBuilder's texel work
+ * and f32_to_f16 (src/pathtrace.rs:1482-1627, src/scene.rs:244-272). */
+typedef struct oxpt_texture_desc {
+    uint32_t w, h;
+    const uint8_t *rgba8;  /* w * h * 4 bytes */
+} oxpt_texture_desc;
+typedef struct oxpt_atlas_chain {
+    uint32_t texture;      /* index into textures */
+    uint32_t srgb;         /* 1: colour channels are sRGB-encoded */
+    uint32_t desc_offset;  /* table word of the chain's texture descriptor */
+    uint32_t pad;
+} oxpt_atlas_chain;
+typedef struct oxpt_atlas_desc {
+    size_t n_textures;
+    const oxpt_texture_desc *textures;
+    size_t n_chains;
+    const oxpt_atlas_chain *chains;
+    const float *decode_lut;  /* 512 floats */
+} oxpt_atlas_desc;
+
 /* Triangle soup for the device build, one record per triangle in input
  * order (the prepared scene's triangles that GpuPathTracer::upload walks,
  * src/pathtrace.rs:971-1118), plus the table and atlas words the caller
@@ -110,6 +136,7 @@ typedef struct oxpt_soup_desc {
     const uint16_t *atlas;        /* binary16 RGBA texels, may be NULL */
     uint32_t atlas_side;
     uint32_t atlas_layers;
+    const oxpt_atlas_desc *atlas_build;  /* instead of atlas: build it on the device (or NULL) */
 } oxpt_soup_desc;
 
 /* What the device build produced: the Params words that depend on the
@@ -120,6 +147,7 @@ typedef struct oxpt_build_info {
     uint32_t pad;
     double upload_ms;  /* soup, tables and atlas host -> device */
     double build_ms;   /* BVH, triangle / attribute gather, 4-wide collapse */
+    double atlas_ms;   /* device atlas (textures uploaded, mips, f16), 0 if none */
 } oxpt_build_info;
 
 /* Builds the scene on the device: the binned-SAH BVH, the triangles and

@@ -10,6 +10,7 @@
 
 #include <functional>
 #include <string>
+#include <vector>
 
 namespace oxd {
 
@@ -55,5 +56,21 @@ constexpr uint32_t kBuildMaxTris = 1u << 24;
 // code with `err` set; temporaries are released either way.
 int build_geometry(const SoupDev &soup, uint32_t n_emissive, uint32_t *emissive_slots, hipStream_t s,
                    const DevAlloc &a, BuiltGeometry *out, std::string &err);
+
+// One mip chain of the device-built texture atlas: level 0 RGBA8 in device
+// memory, its level count, the levels dropped before the first kept one,
+// and the rectangles (layer, x, y, w | h << 16) of the kept levels.
+struct AtlasChainDev {
+    const uint8_t *rgba8 = nullptr;
+    uint32_t w = 0, h = 0, srgb = 0, levels = 0, skip = 0;
+    std::vector<uint32_t> rects;  // 4 words per kept level
+};
+
+// Decodes, filters and converts every chain into `atlas` (side x side x
+// layers RGBA binary16, cleared first) on stream `s`; lut: 512 device
+// floats (the host texel_decode_lut). Rectangles must lie inside the
+// atlas and match the level sizes (checked by the caller).
+int build_atlas(const std::vector<AtlasChainDev> &chains, const float *lut, uint16_t *atlas, uint32_t side,
+                uint32_t layers, hipStream_t s, const DevAlloc &a, std::string &err);
 
 }  // namespace oxd

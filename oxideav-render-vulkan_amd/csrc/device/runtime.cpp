@@ -347,6 +347,92 @@ int wf_run(oxpt_ctx *c, const std::vector<oxd::Job> &jobs) {
 
 }  // namespace
 
+namespace {
+
+// Checks every chain of an atlas build against the tables' texture
+// descriptors and the atlas shape, uploads the textures and the decode
+// table, and runs the device atlas builder into `atlas`.
+int build_atlas_on_device(oxpt_ctx *c, const oxpt_atlas_desc &ad, const std::vector<uint32_t> &tables,
+                          uint16_t *atlas, uint32_t side, uint32_t layers) {
+    if (!ad.decode_lut || (ad.n_chains && (!ad.chains || !ad.textures)))
+        return fail(c, OXPT_E_ARG, "atlas build: chains, textures and the decode table are required");
+    std::vector<oxd::AtlasChainDev> chains(ad.n_chains);
+    std::vector<int> used(ad.n_textures, 0);
+    for (size_t i = 0; i < ad.n_chains; i++) {
+        const oxpt_atlas_chain &ch = ad.chains[i];
+        if (ch.texture >= ad.n_textures) return fail(c, OXPT_E_ARG, "atlas build: texture index out of range");
+        const oxpt_texture_desc &tx = ad.textures[ch.texture];
+        if (!tx.rgba8 || tx.w == 0 || tx.h == 0) return fail(c, OXPT_E_ARG, "atlas build: empty texture");
+        if ((size_t)ch.desc_offset + 8 > tables.size()) return fail(c, OXPT_E_ARG, "atlas build: descriptor outside the tables");
+        const uint32_t *dw = &tables[ch.desc_offset];
+        uint32_t levels = 1;
+        for (uint32_t w = tx.w, h = tx.h; w > 1 || h > 1; levels++) {
+            w = std::max(1u, w / 2);
+            h = std::max(1u, h / 2);
+        }
+        if (dw[0] != levels || dw[2] != tx.w || dw[3] != tx.h || dw[4] > levels || dw[5] != levels - dw[4] ||
+            (size_t)ch.desc_offset + 8 + 4ull * dw[5] > tables.size())
+            return fail(c, OXPT_E_ARG, "atlas build: descriptor does not match its texture");
+        oxd::AtlasChainDev &cd = chains[i];
+        cd.w = tx.w;
+        cd.h = tx.h;
+        cd.srgb = ch.srgb ? 1u : 0u;
+        cd.levels = levels;
+        cd.skip = dw[4];
+        cd.rects.assign(dw + 8, dw + 8 + 4 * dw[5]);
+        uint32_t w = tx.w, h = tx.h;
+        for (uint32_t li = 0; li < levels; li++) {
+            if (li >= cd.skip) {
+                const uint32_t *r = &cd.rects[4 * (li - cd.skip)];
+                if (r[0] >= layers || r[3] != (w | h << 16) || (uint64_t)r[1] + w > side || (uint64_t)r[2] + h > side)
+                    return fail(c, OXPT_E_ARG, "atlas build: level rectangle outside the atlas");
+            }
+            w = std::max(1u, w / 2);
+            h = std::max(1u, h / 2);
+        }
+        used[ch.texture] = 1;
+    }
+    std::vector<void *> held;
+    std::vector<size_t> held_bytes;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(c->stream);
+        for (size_t i = 0; i < held.size(); i++) dfree(c, held[i], held_bytes[i]);
+    };
+    std::vector<const uint8_t *> dev_tex(ad.n_textures, nullptr);
+    int rc = OXPT_OK;
+    for (size_t t = 0; t < ad.n_textures && rc == OXPT_OK; t++) {
+        if (!used[t]) continue;
+        const size_t bytes = (size_t)ad.textures[t].w * ad.textures[t].h * 4;
+        void *p = nullptr;
+        if ((rc = dalloc(c, bytes, &p, "atlas build: textures"))) break;
+        held.push_back(p);
+        held_bytes.push_back(bytes);
+        dev_tex[t] = (const uint8_t *)p;
+        hipError_t e = hipMemcpyAsync(p, ad.textures[t].rgba8, bytes, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, "atlas build: texture upload", e);
+    }
+    void *lut = nullptr;
+    if (rc == OXPT_OK && (rc = dalloc(c, 512 * 4, &lut, "atlas build: decode table")) == OXPT_OK) {
+        held.push_back(lut);
+        held_bytes.push_back(512 * 4);
+        hipError_t e = hipMemcpyAsync(lut, ad.decode_lut, 512 * 4, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, "atlas build: decode table upload", e);
+    }
+    if (rc == OXPT_OK) {
+        for (size_t i = 0; i < chains.size(); i++) chains[i].rgba8 = dev_tex[ad.chains[i].texture];
+        oxd::DevAlloc al;
+        al.alloc = [c](size_t bytes, void **p, const char *what) { return dalloc(c, bytes, p, what); };
+        al.release = [c](void *p, size_t bytes) { dfree(c, p, bytes); };
+        std::string err;
+        if ((rc = oxd::build_atlas(chains, (const float *)lut, atlas, side, layers, c->stream, al, err)))
+            fail(c, rc, err);
+    }
+    release();
+    return rc;
+}
+
+}  // namespace
+
 extern "C" {
 
 int oxpt_ctx_create(int ordinal, oxpt_ctx **out) {
@@ -537,6 +623,7 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
         return fail(c, OXPT_E_INVALID, "scene build: the device builder takes 1 to 16777216 triangles");
     if (d->n_emissive > 0 && (!d->light_of || (size_t)d->elight_offset + 4ull * d->n_emissive > d->table_words))
         return fail(c, OXPT_E_ARG, "scene build: emissive records need light_of and fit the tables");
+    if (d->atlas && d->atlas_build) return fail(c, OXPT_E_ARG, "scene build: atlas texels or atlas_build, not both");
     const size_t n = d->n_tris;
     uint32_t any = 0;
     for (size_t i = 0; i < n; i++) any |= d->flags[i];
@@ -629,20 +716,27 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
     s->atlas_side = std::max(d->atlas_side, 1u);
     s->atlas_layers = std::max(d->atlas_layers, 1u);
     const size_t tb = tables.size() * 4;
-    const size_t xb = d->atlas ? (size_t)s->atlas_side * s->atlas_side * s->atlas_layers * 8 : 8;
+    const size_t xb =
+        (d->atlas || d->atlas_build) ? (size_t)s->atlas_side * s->atlas_side * s->atlas_layers * 8 : 8;
     s->atlas_words = xb / 2;
     s->bytes = bg.geom_vec4 * 16 + bg.attrs_vec4 * 16 + tb + xb;
     void *p = nullptr;
     if (rc == OXPT_OK && (rc = dalloc(c, tb, &p, "scene tables")) == OXPT_OK) s->tables = (uint32_t *)p;
     if (rc == OXPT_OK && (rc = dalloc(c, xb, &p, "texture atlas")) == OXPT_OK) s->atlas = (uint16_t *)p;
+    double atlas_ms = 0.0;
     if (rc == OXPT_OK) {
         s->host_tables = tables;
         hipError_t e = hipMemcpyAsync(s->tables, tables.data(), tb, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess)
+        if (e == hipSuccess && !d->atlas_build)
             e = d->atlas ? hipMemcpyAsync(s->atlas, d->atlas, xb, hipMemcpyHostToDevice, c->stream)
                          : hipMemsetAsync(s->atlas, 0, xb, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = hip_fail(c, "scene build: tables / atlas upload", e);
+    }
+    if (rc == OXPT_OK && d->atlas_build) {
+        const auto ta = std::chrono::steady_clock::now();
+        rc = build_atlas_on_device(c, *d->atlas_build, tables, s->atlas, s->atlas_side, s->atlas_layers);
+        atlas_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
     }
     if (rc) {
         // Account the build's buffers to the scene so destroy releases them.
@@ -661,6 +755,7 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
         info->pad = 0;
         info->upload_ms = upload_ms;
         info->build_ms = bg.ms;
+        info->atlas_ms = atlas_ms;
     }
     *out = s;
     return OXPT_OK;

@@ -65,6 +65,14 @@ std::vector<Mip> build_mips(const Texture &tx, bool srgb) {
     return out;
 }
 
+// Level sizes of build_mips' chain: halve (at least 1) down to 1x1.
+std::vector<std::pair<uint32_t, uint32_t>> mip_dims(uint32_t w, uint32_t h) {
+    std::vector<std::pair<uint32_t, uint32_t>> d{{w, h}};
+    while (d.back().first > 1 || d.back().second > 1)
+        d.push_back({std::max(1u, d.back().first / 2), std::max(1u, d.back().second / 2)});
+    return d;
+}
+
 uint32_t sampler_flags(const oxpt_sampler &s) {
     return (s.wrap_s & 3u) | (s.wrap_t & 3u) << 2 | (s.mag & 1u) << 4 | (s.min & 7u) << 5;
 }
@@ -212,6 +220,13 @@ void sheen_table(float out[OXPT_SHEEN_WORDS]) {
     std::memcpy(out, lut.data(), OXPT_SHEEN_WORDS * sizeof(float));
 }
 
+void texel_decode_lut(float out[512]) {
+    for (int i = 0; i < 256; i++) {
+        out[i] = srgb_to_linear((float)i / 255.0f);
+        out[256 + i] = (float)i / 255.0f;
+    }
+}
+
 uint16_t f32_to_f16(float v) {
     // binary32 -> binary16, round to nearest even, overflow to infinity,
     // subnormals kept, NaN stays NaN (src/scene.rs:244-272).
@@ -308,7 +323,7 @@ CameraFrame resolve_camera(const CameraSource &s, uint32_t w, uint32_t h) {
     return f;
 }
 
-Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *light_of_out) {
+Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *light_of_out, bool atlas) {
     Packed pk;
     const Soup &soup = s.soup;
     size_t ntri = soup.count();
@@ -440,25 +455,32 @@ Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *ligh
         }
     std::vector<std::pair<int, bool>> keys;
     for (auto &kv : descs) keys.push_back(kv.first);
+    // Placement needs only the level sizes; the texels are decoded,
+    // filtered and converted here (atlas) or by the device builder.
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> dims;
     std::vector<std::vector<Mip>> chains;
-    for (auto &k : keys) chains.push_back(build_mips(s.textures[k.first], k.second));
+    for (auto &k : keys) {
+        dims.push_back(mip_dims(s.textures[k.first].w, s.textures[k.first].h));
+        if (atlas) chains.push_back(build_mips(s.textures[k.first], k.second));
+    }
     uint32_t biggest = 1;
-    for (auto &c : chains) biggest = std::max(biggest, std::max(c[0].w, c[0].h));
+    for (auto &c : dims) biggest = std::max(biggest, std::max(c[0].first, c[0].second));
     uint32_t side = 1;
     while (side < biggest) side <<= 1;
     side = std::min(std::max(side, 1u), OXPT_MAX_LAYER);
-    std::vector<Placed> placed(chains.size());
+    std::vector<Placed> placed(dims.size());
     struct Item {
         size_t ci, li;
         uint32_t w, h;
     };
     std::vector<Item> items;
-    for (size_t ci = 0; ci < chains.size(); ci++) {
+    for (size_t ci = 0; ci < dims.size(); ci++) {
         uint32_t skip = 0;
-        while (skip < chains[ci].size() && (chains[ci][skip].w > side || chains[ci][skip].h > side)) skip++;
+        while (skip < dims[ci].size() && (dims[ci][skip].first > side || dims[ci][skip].second > side)) skip++;
         placed[ci].skip = skip;
-        placed[ci].levels = (uint32_t)chains[ci].size();
-        for (size_t li = skip; li < chains[ci].size(); li++) items.push_back({ci, li, chains[ci][li].w, chains[ci][li].h});
+        placed[ci].levels = (uint32_t)dims[ci].size();
+        for (size_t li = skip; li < dims[ci].size(); li++)
+            items.push_back({ci, li, dims[ci][li].first, dims[ci][li].second});
     }
     std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
         return a.h != b.h ? a.h > b.h : a.w > b.w;
@@ -483,20 +505,24 @@ Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *ligh
     }
     std::sort(rects.begin(), rects.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
     pk.atlas_side = side;
-    pk.atlas_layers = chains.empty() ? 1u : layer + 1;
-    pk.atlas.assign((size_t)pk.atlas_layers * side * side * 4, 0);
+    pk.atlas_layers = dims.empty() ? 1u : layer + 1;
+    if (atlas) pk.atlas.assign((size_t)pk.atlas_layers * side * side * 4, 0);
     for (auto &pr : rects) {
-        const Mip &m = chains[pr.first.first][pr.first.second];
         const Rect &r = pr.second;
-        for (uint32_t row = 0; row < r.h; row++)
-            for (uint32_t col = 0; col < r.w; col++) {
-                size_t at = (((size_t)r.layer * side + r.y + row) * side + r.x + col) * 4;
-                for (int c = 0; c < 4; c++) pk.atlas[at + c] = f32_to_f16(m.t[4 * ((size_t)row * r.w + col) + c]);
-            }
+        if (atlas) {
+            const Mip &m = chains[pr.first.first][pr.first.second];
+            for (uint32_t row = 0; row < r.h; row++)
+                for (uint32_t col = 0; col < r.w; col++) {
+                    size_t at = (((size_t)r.layer * side + r.y + row) * side + r.x + col) * 4;
+                    for (int c = 0; c < 4; c++) pk.atlas[at + c] = f32_to_f16(m.t[4 * ((size_t)row * r.w + col) + c]);
+                }
+        }
         placed[pr.first.first].rects.push_back(r);
     }
+    pk.chains.clear();
     for (size_t ci = 0; ci < keys.size(); ci++) {
         descs[keys[ci]] = (uint32_t)t.size();
+        pk.chains.push_back({(uint32_t)keys[ci].first, keys[ci].second ? 1u : 0u, (uint32_t)t.size(), 0u});
         const Texture &tx = s.textures[keys[ci].first];
         t.insert(t.end(), {placed[ci].levels, sampler_flags(tx.sampler), tx.w, tx.h, placed[ci].skip,
                            (uint32_t)placed[ci].rects.size(), 0u, 0u});

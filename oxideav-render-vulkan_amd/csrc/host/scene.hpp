@@ -80,6 +80,9 @@ struct Packed {
     uint32_t bvh_depth = 0;
     uint32_t wide_base = 0;   // first float4 of the 4-wide nodes in geom (0: none)
     uint32_t wide_depth = 0;
+    // One mip chain per (texture, colour space) a material slot binds:
+    // {texture, srgb, table word of its descriptor, 0} (oxpt_atlas_chain).
+    std::vector<oxpt_atlas_chain> chains;
 };
 
 struct EnvWords {
@@ -104,7 +107,11 @@ uint32_t build_wide(const std::vector<uint32_t> &node_words, std::vector<float> 
 // device builder (oxpt_scene_build): geom / attrs stay empty and the
 // emissive records' slot words hold OXPT_NONE. light_of_out receives each
 // triangle's emissive-light index (OXPT_NONE for none).
-Packed pack_scene(const HostScene &s, bool geometry = true, std::vector<uint32_t> *light_of_out = nullptr);
+// With atlas = false the texels are left to the device builder
+// (oxpt_atlas_desc): descriptors and placement are written, pk.atlas stays
+// empty.
+Packed pack_scene(const HostScene &s, bool geometry = true, std::vector<uint32_t> *light_of_out = nullptr,
+                  bool atlas = true);
 // Environment section (env_words, src/pathtrace.rs:1428-1461).
 EnvWords env_words(const EnvMap &e);
 
@@ -116,6 +123,8 @@ void bind(const Packed &pk, const EnvMap *env, const CameraFrame &cam, const oxp
 void sobol_directions(uint32_t out[OXPT_SOBOL_WORDS]);
 void sheen_table(float out[OXPT_SHEEN_WORDS]);
 uint16_t f32_to_f16(float v);
+// Level-0 texel decode: [0, 256) sRGB EOTF of i / 255, [256, 512) i / 255.
+void texel_decode_lut(float out[512]);
 
 }  // namespace oxpt
 

@@ -70,7 +70,7 @@ struct oxpt_tracer {
 
     int upload_device(const oxpt::HostScene &s) {
         std::vector<uint32_t> light_of;
-        auto pk = std::make_unique<oxpt::Packed>(oxpt::pack_scene(s, false, &light_of));
+        auto pk = std::make_unique<oxpt::Packed>(oxpt::pack_scene(s, false, &light_of, false));
         if (dscene) {
             oxpt_scene_destroy(dscene);
             dscene = nullptr;
@@ -91,9 +91,17 @@ struct oxpt_tracer {
         d.elight_offset = pk->off_elights;
         d.tables = pk->tables.data();
         d.table_words = pk->tables.size();
-        d.atlas = pk->atlas.empty() ? nullptr : pk->atlas.data();
+        d.atlas = nullptr;
         d.atlas_side = pk->atlas_side;
         d.atlas_layers = pk->atlas_layers;
+        // Texels: decoded, filtered and converted on the device.
+        std::vector<oxpt_texture_desc> tex(s.textures.size());
+        for (size_t i = 0; i < s.textures.size(); i++)
+            tex[i] = {s.textures[i].w, s.textures[i].h, s.textures[i].rgba8.empty() ? nullptr : s.textures[i].rgba8.data()};
+        float lut[512];
+        oxpt::texel_decode_lut(lut);
+        oxpt_atlas_desc ad{tex.size(), tex.data(), pk->chains.size(), pk->chains.data(), lut};
+        d.atlas_build = pk->chains.empty() ? nullptr : &ad;
         std::vector<uint32_t> slots(pk->n_emissive);
         int rc = oxpt_scene_build(ctx, &d, slots.data(), &dscene, &build_info);
         if (rc) return dev_fail(rc);
@@ -119,6 +127,13 @@ struct oxpt_tracer {
         packed->attrs.resize(av * 4);
         rc = oxpt_scene_read(ctx, dscene, 0, packed->geom.data(), gv * 16, nullptr);
         if (!rc) rc = oxpt_scene_read(ctx, dscene, 1, packed->attrs.data(), av * 16, nullptr);
+        if (rc) return dev_fail(rc);
+        size_t xv = 0;
+        rc = oxpt_scene_read(ctx, dscene, 3, nullptr, 0, &xv);
+        if (!rc) {
+            packed->atlas.resize(xv);
+            rc = oxpt_scene_read(ctx, dscene, 3, packed->atlas.data(), xv * 2, nullptr);
+        }
         if (rc) return dev_fail(rc);
         words_on_device = false;
         return OXPT_OK;
@@ -422,7 +437,7 @@ int oxpt_tracer_words(oxpt_tracer *t, int which, const void **ptr, size_t *count
     if (!t || !ptr || !count) return OXPT_E_ARG;
     if (!t->packed) return t->fail(OXPT_E_ARG, "words: not synced");
     if (int rc = t->ensure_binding()) return rc;
-    if (which <= 1)
+    if (which <= 1 || which == 3)
         if (int rc = t->fetch_words()) return rc;
     switch (which) {
         case 0: *ptr = t->packed->geom.data(); *count = t->packed->geom.size() / 4; return OXPT_OK;

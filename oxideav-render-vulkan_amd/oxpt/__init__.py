@@ -88,12 +88,14 @@ class SoupDesc(C.Structure):
         ("uv1", P(f32)), ("colors", P(f32)), ("material", P(u32)), ("flags", P(C.c_uint8)),
         ("light_of", P(u32)), ("n_emissive", u32), ("elight_offset", u32), ("tables", P(u32)),
         ("table_words", sz), ("atlas", P(C.c_uint16)), ("atlas_side", u32), ("atlas_layers", u32),
+        ("atlas_build", C.c_void_p),
     ]
 
 
 class BuildInfo(C.Structure):
     _fields_ = [("node_count", u32), ("tri_base", u32), ("wide_base", u32), ("bvh_depth", u32),
-                ("wide_depth", u32), ("pad", u32), ("upload_ms", C.c_double), ("build_ms", C.c_double)]
+                ("wide_depth", u32), ("pad", u32), ("upload_ms", C.c_double), ("build_ms", C.c_double),
+                ("atlas_ms", C.c_double)]
 
 
 class TexBinding(C.Structure):

@@ -1,6 +1,7 @@
 """Device scene preparation (oxpt_scene_build): the BVH, the leaf-ordered
-triangle / attribute words and the 4-wide nodes built on the GPU are the
-host packer's words bit for bit, and so the images are the same.
+triangle / attribute words, the 4-wide nodes and the f16 texture atlas
+built on the GPU are the host packer's words bit for bit, and so the
+images are the same.
 
 The reference prepares its scene on the CPU (GpuPathTracer::upload,
 src/pathtrace.rs:970-1239, with oxideav-mesh3d's Bvh); the host packer is
@@ -59,7 +60,31 @@ def flat_grid(n):
     return s
 
 
+def textures_odd():
+    """Odd-sized textures bound in both colour spaces (sRGB base colour,
+    linear metallic-roughness / normal), and one wider than the largest
+    atlas layer, whose level 0 is dropped."""
+    rng = np.random.default_rng(11)
+    s = oxpt.Scene()
+    tex = [s.add_texture(rng.integers(0, 256, (h, w, 4), dtype=np.uint8), smp)
+           for (w, h), smp in (((37, 19), (2, 2, 1, 5)), ((5, 300), (0, 1, 0, 2)), ((5000, 4), (2, 2, 1, 4)))]
+    mats = []
+    for i, t in enumerate(tex):
+        m = oxpt.material(base_color=(0.8, 0.7, 0.6, 1.0), metallic=0.1, roughness=0.6)
+        m.tex[0].texture = t          # base colour: sRGB
+        m.tex[2].texture = t          # metallic-roughness: linear
+        m.tex[1].texture = tex[(i + 1) % 3]  # normal map: linear
+        mats.append(s.add_material(m))
+    pos, uv = ptscenes.quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1))
+    for i, m in enumerate(mats):
+        s.add_triangles(pos + np.float32(0.1 * i), m, uv0=uv)
+    return s
+
+
 SCENES = {
+    "textures_odd": textures_odd,
+    "checker": ptscenes.checker_floor,
+    "alpha": ptscenes.alpha_planes,
     "room": lambda: oxpt.Scene.room(0),
     "single": lambda: coincident(1),
     "coincident_small": lambda: coincident(300),
@@ -99,8 +124,14 @@ def test_device_build_writes_the_host_words(name):
         hb, db = host.view(np.uint32), dev.view(np.uint32)
         bad = np.flatnonzero(hb != db)
         assert bad.size == 0, f"{name}: buffer {which} differs at {bad[:8]} ({bad.size} words)"
+    dev = pt.scene_words(3)
+    assert dev.size == pk.atlas.size
+    bad = np.flatnonzero(dev != pk.atlas)
+    assert bad.size == 0, f"{name}: atlas differs at {bad[:8]} ({bad.size} texel words)"
+    assert (info.atlas_ms > 0.0) == (pk.atlas.size > 4)  # a scene without textures keeps one blank texel
     # The host copies the tracer hands the oracle are the device words.
     assert np.array_equal(pt.words(0).view(np.uint32), pk.geom.view(np.uint32))
+    assert np.array_equal(pt.words(3), pk.atlas)
 
 
 def test_device_built_scene_renders_the_same_image():
