@@ -83,6 +83,22 @@ int oxpt_scene_upload(oxpt_ctx *ctx, const oxpt_scene_desc *desc, oxpt_scene **o
 /* Replace the table words only (environment change, ensure_binding,
  * src/pathtrace.rs:847-869); geometry and atlas stay resident. */
 int oxpt_scene_set_tables(oxpt_ctx *ctx, oxpt_scene *scene, const uint32_t *tables, size_t words);
+/* Equirectangular environment for the device-built sampling tables. */
+typedef struct oxpt_env_desc {
+    uint32_t w, h;
+    const float *rgb;          /* w * h * 3 floats */
+    float intensity;
+    uint32_t pad;
+    const double *row_weight;  /* h values: sin(pi (y + 0.5) / h) */
+} oxpt_env_desc;
+/* As oxpt_scene_set_tables, then the environment section built on the
+ * device right after the `words` table words: marginal CDF (h + 1),
+ * conditional CDFs (h x (w + 1)), texel mass (w h), radiance (3 w h);
+ * texel weight = luminance(max(rgb, 0) x intensity) x row_weight[y] in
+ * f64, sums in row order (env_words, src/pathtrace.rs:1428-1461; used by
+ * env_sample / env_pdf, src/shaders/pathtrace.wgsl:1594-1654). */
+int oxpt_scene_set_tables_env(oxpt_ctx *ctx, oxpt_scene *scene, const uint32_t *tables, size_t words,
+                              const oxpt_env_desc *env);
 void oxpt_scene_destroy(oxpt_scene *scene);
 /* Device bytes held by a scene. */
 uint64_t oxpt_scene_bytes(const oxpt_scene *scene);
@@ -137,6 +153,14 @@ typedef struct oxpt_soup_desc {
     uint32_t atlas_side;
     uint32_t atlas_layers;
     const oxpt_atlas_desc *atlas_build;  /* instead of atlas: build it on the device (or NULL) */
+    /* Emissive lights found on the device (EmissiveLights::build,
+     * src/pathtrace.rs:1201-1210): per material the luminance of its
+     * emissive factor (0: does not emit). Every triangle with area x that
+     * luminance > 0 (area in f64) is a light, indexed in input order; the
+     * CDF over those weights and the records {cdf, pmf, triangle, slot} are
+     * appended after the tables (light_of must be NULL, n_emissive 0). */
+    const float *material_emission;
+    uint32_t n_materials;
 } oxpt_soup_desc;
 
 /* What the device build produced: the Params words that depend on the
@@ -148,6 +172,8 @@ typedef struct oxpt_build_info {
     double upload_ms;  /* soup, tables and atlas host -> device */
     double build_ms;   /* BVH, triangle / attribute gather, 4-wide collapse */
     double atlas_ms;   /* device atlas (textures uploaded, mips, f16), 0 if none */
+    uint32_t n_emissive;     /* emissive lights (found on the device or given) */
+    uint32_t elight_offset;  /* table word of emissive record 0 */
 } oxpt_build_info;
 
 /* Builds the scene on the device: the binned-SAH BVH, the triangles and

@@ -1,8 +1,12 @@
-// Device texture atlas for gfx950: level-0 decode, 2x2 box-filtered mips
-// and binary16 conversion into the atlas layers, the host packer's texels
-// bit for bit (csrc/host/pack.cpp build_mips, f32_to_f16). The reference
-// builds its Rgba16Float atlas on the CPU (AtlasBuilder,
-// src/pathtrace.rs:1482-1627; f32_to_f16, src/scene.rs:244-272).
+// Device-built tables for gfx950, the host packer's words bit for bit:
+//  - the texture atlas: level-0 decode, 2x2 box-filtered mips and binary16
+//    conversion into the atlas layers (csrc/host/pack.cpp build_mips,
+//    f32_to_f16; the reference's AtlasBuilder, src/pathtrace.rs:1482-1627,
+//    and f32_to_f16, src/scene.rs:244-272);
+//  - the environment sampling tables (pack.cpp env_words; the reference's
+//    env_words, src/pathtrace.rs:1428-1461): texel weights in parallel,
+//    every sum in the host's order (one thread per row, one for the
+//    marginal), in f64.
 //
 // Levels are produced one at a time into two float4 scratch buffers; each
 // kernel is one thread per texel, reads are coalesced row-major and the
@@ -76,9 +80,96 @@ __global__ void k_store(const float4 *lv, uint32_t w, uint32_t h, uint2 *atlas, 
         make_uint2(to_f16(t.x) | to_f16(t.y) << 16, to_f16(t.z) | to_f16(t.w) << 16);
 }
 
+// ---- environment sampling tables (env_words) ----------------------------------
+
+__device__ inline float lum(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
+
+// Per texel: radiance (max(rgb, 0) x intensity, 0 when any channel is not
+// finite) and weight luminance x row weight (f64, negatives to 0).
+__global__ void k_env_texel(const float *rgb, uint32_t w, uint32_t h, float intensity, const double *row_w,
+                            double *wt, uint32_t *rad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const float *q = rgb + 3 * (size_t)i;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]))
+        for (int k = 0; k < 3; k++) c[k] = (q[k] < 0.0f ? 0.0f : q[k]) * intensity;  // std::max(q, 0.0f)
+    for (int k = 0; k < 3; k++) rad[3 * (size_t)i + k] = __float_as_uint(c[k]);
+    const double l = (double)lum(c[0], c[1], c[2]) * row_w[i / w];
+    wt[i] = l > 0.0 ? l : 0.0;
+}
+
+// One thread per row: the row's weight sum in column order.
+__global__ void k_env_rows(const double *wt, uint32_t w, uint32_t h, double *rows) {
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= h) return;
+    double r = 0.0;
+    for (uint32_t x = 0; x < w; x++) r += wt[(size_t)y * w + x];
+    rows[y] = r;
+}
+
+// One thread: the total in row order, the marginal CDF and each row's
+// probability (uniform rows when the map has no positive weight).
+__global__ void k_env_marginal(const double *rows, uint32_t h, uint32_t *marg, double *rowp, uint32_t *uniform) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double total = 0.0;
+    for (uint32_t y = 0; y < h; y++) total += rows[y];
+    const bool uni = !(total > 0.0);
+    double run = 0.0;
+    marg[0] = __float_as_uint(0.0f);
+    for (uint32_t y = 0; y < h; y++) {
+        const double p = uni ? 1.0 / h : rows[y] / total;
+        run += p;
+        rowp[y] = p;
+        marg[y + 1] = __float_as_uint(y + 1 == h ? 1.0f : (float)run);
+    }
+    *uniform = uni ? 1u : 0u;
+}
+
+// One thread per row: the conditional CDF in column order and the texel
+// masses (row probability x column probability).
+__global__ void k_env_conditional(const double *wt, const double *rows, const double *rowp, const uint32_t *uniform,
+                                  uint32_t w, uint32_t h, uint32_t *cond, uint32_t *mass) {
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= h) return;
+    const bool row_uniform = *uniform || !(rows[y] > 0.0);
+    uint32_t *c = cond + (size_t)y * (w + 1);
+    c[0] = __float_as_uint(0.0f);
+    double rr = 0.0;
+    for (uint32_t x = 0; x < w; x++) {
+        const double p = row_uniform ? 1.0 / w : wt[(size_t)y * w + x] / rows[y];
+        rr += p;
+        c[x + 1] = __float_as_uint(x + 1 == w ? 1.0f : (float)rr);
+        mass[(size_t)y * w + x] = __float_as_uint((float)(rowp[y] * p));
+    }
+}
+
 inline uint32_t blocks(size_t n) { return (uint32_t)((n + 255) / 256); }
 
 }  // namespace
+
+int build_env(const float *rgb, uint32_t w, uint32_t h, float intensity, const double *row_w, uint32_t *out,
+              hipStream_t s, const DevAlloc &al, std::string &err) {
+    const size_t n = (size_t)w * h;
+    const size_t bytes = (2 * n + 2 * h + 1) * 8;
+    void *scratch = nullptr;
+    if (int rc = al.alloc(bytes, &scratch, "environment build: weights")) return rc;
+    double *wt = static_cast<double *>(scratch), *rows = wt + n, *rowp = rows + h;
+    uint32_t *uniform = reinterpret_cast<uint32_t *>(rowp + h);
+    uint32_t *marg = out, *cond = marg + (h + 1), *mass = cond + (size_t)h * (w + 1), *rad = mass + n;
+    k_env_texel<<<blocks(n), 256, 0, s>>>(rgb, w, h, intensity, row_w, wt, rad);
+    k_env_rows<<<blocks(h), 256, 0, s>>>(wt, w, h, rows);
+    k_env_marginal<<<1, 64, 0, s>>>(rows, h, marg, rowp, uniform);
+    k_env_conditional<<<blocks(h), 256, 0, s>>>(wt, rows, rowp, uniform, w, h, cond, mass);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    al.release(scratch, bytes);
+    if (e != hipSuccess) {
+        err = std::string("hip: device environment build: ") + hipGetErrorString(e);
+        return OXPT_E_BACKEND;
+    }
+    return 0;
+}
 
 int build_atlas(const std::vector<AtlasChainDev> &chains, const float *lut, uint16_t *atlas, uint32_t side,
                 uint32_t layers, hipStream_t s, const DevAlloc &al, std::string &err) {

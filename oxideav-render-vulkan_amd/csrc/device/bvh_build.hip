@@ -880,6 +880,68 @@ __global__ void k_emissive_slots(const uint32_t *ids, const uint32_t *light_of, 
     if (li < n_emissive) out[li] = sl;
 }
 
+// ---- emissive lights (EmissiveLights::build) --------------------------------
+
+// Weight of triangle t: area (f64 cross product) x its material's emission
+// luminance, the host packer's arithmetic; flag = weight > 0.
+__global__ void k_emit_weight(const float *pos, const uint32_t *material, const float *mat_lum, uint32_t n_mat,
+                              uint32_t n, double *wgt, uint32_t *flag) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t m = material[t];
+    const float lum = m < n_mat ? mat_lum[m] : 0.0f;
+    double w = 0.0;
+    if (lum != 0.0f) {
+        const float *p = pos + 9 * (size_t)t;
+        double e1[3], e2[3];
+        for (int k = 0; k < 3; k++) {
+            e1[k] = (double)p[3 + k] - p[k];
+            e2[k] = (double)p[6 + k] - p[k];
+        }
+        const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                     cz = e1[0] * e2[1] - e1[1] * e2[0];
+        const double area = 0.5 * __builtin_sqrt(cx * cx + cy * cy + cz * cz);
+        w = area * (double)lum;
+    }
+    wgt[t] = w;
+    flag[t] = w > 0.0 ? 1u : 0u;
+}
+
+// rank: exclusive prefix of the flags. Light index = rank, in input order.
+__global__ void k_light_of(const uint32_t *flag, const uint32_t *rank, const double *wgt, uint32_t n,
+                           uint32_t *light_of, double *wgt_c, uint32_t *tri_of, uint32_t *count) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (flag[t]) {
+        light_of[t] = rank[t];
+        wgt_c[rank[t]] = wgt[t];
+        tri_of[rank[t]] = t;
+    } else {
+        light_of[t] = OXPT_NONE;
+    }
+    if (t == n - 1) *count = rank[t] + flag[t];
+}
+
+// Records {cdf, pmf, triangle, slot}: the CDF over the weights in light
+// order, sums in that order (one thread), the last entry exactly 1.
+__global__ void k_emissive_records(const double *wgt_c, const uint32_t *tri_of, const uint32_t *slots, uint32_t n,
+                                   uint32_t *rec) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double total = 0.0;
+    for (uint32_t i = 0; i < n; i++) total += wgt_c[i];
+    double run = 0.0;
+    float prev = 0.0f;
+    for (uint32_t i = 0; i < n; i++) {
+        run += wgt_c[i];
+        const float cdf = i + 1 == n ? 1.0f : (float)(run / total);
+        rec[4 * i] = __float_as_uint(cdf);
+        rec[4 * i + 1] = __float_as_uint(cdf - prev);
+        rec[4 * i + 2] = tri_of[i];
+        rec[4 * i + 3] = slots[i];
+        prev = cdf;
+    }
+}
+
 // ---- 5. 4-wide collapse ------------------------------------------------------
 
 __device__ inline float wide_area(const uint32_t *words, uint32_t node) {
@@ -1062,6 +1124,31 @@ int build_geometry(const SoupDev &soup, uint32_t n_emissive, uint32_t *emissive_
         return rc;
     uint32_t hc[C_COUNT];
 
+    // Emissive lights found here (before the gather, which writes light_of).
+    SoupDev sd = soup;
+    uint32_t found = 0, *found_slots = nullptr, *tri_of = nullptr;
+    double *wgt_c = nullptr;
+    if (soup.mat_lum) {
+        double *wgt;
+        uint32_t *flag, *rank, *lof;
+        if ((rc = sc.get(N, &wgt, "BVH build: light weights")) || (rc = sc.get(N, &wgt_c, "BVH build: light weights")) ||
+            (rc = sc.get(N, &flag, "BVH build: lights")) || (rc = sc.get(N, &rank, "BVH build: lights")) ||
+            (rc = sc.get(N, &lof, "BVH build: lights")) || (rc = sc.get(N, &tri_of, "BVH build: lights")))
+            return rc;
+        k_emit_weight<<<blocks(N, 256), 256, 0, s>>>(soup.pos, soup.material, soup.mat_lum, soup.n_mat, N, wgt, flag);
+        hipError_t e = hipMemcpyAsync(rank, flag, 4ull * N, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            err = std::string("hip: device light search: ") + hipGetErrorString(e);
+            return OXPT_E_BACKEND;
+        }
+        if ((rc = scan(rank, N, sums, s, err))) return rc;
+        k_light_of<<<blocks(N, 256), 256, 0, s>>>(flag, rank, wgt, N, lof, wgt_c, tri_of, counters + C_WIDE_BAD);
+        if ((rc = launch_check(err, "light search"))) return rc;
+        if ((rc = read_counters(counters, hc, s, err))) return rc;
+        found = hc[C_WIDE_BAD];
+        sd.light_of = lof;
+        if (found > 0 && (rc = sc.get(found, &found_slots, "BVH build: light slots"))) return rc;
+    }
     k_prep<<<blocks(N, 256), 256, 0, s>>>(soup.pos, N, tlo, thi, tcz, ids);
     k_init<<<1, 1, 0, s>>>(nodes, N, counters, open, small);
     k_root_bounds<<<std::min(1024u, blocks(N, 256)), 256, 0, s>>>(tlo, thi, tcz, N, nodes);
@@ -1173,7 +1260,7 @@ int build_geometry(const SoupDev &soup, uint32_t n_emissive, uint32_t *emissive_
         e = hipMemsetAsync(out->geom + tri_end, 0, (out->wide_base - tri_end) * 16, s);
         if (e != hipSuccess) return fail_out("pad", e);
     }
-    k_gather<<<blocks(N, 256), 256, 0, s>>>(soup, ids, out->geom + out->tri_base, out->attrs);
+    k_gather<<<blocks(N, 256), 256, 0, s>>>(sd, ids, out->geom + out->tri_base, out->attrs);
     if (wide_ok) {
         k_fill<<<blocks(N, 256), 256, 0, s>>>(cnt, N, 0u);
         k_fill<<<blocks(N, 256), 256, 0, s>>>(d0, N, kNone);
@@ -1190,10 +1277,26 @@ int build_geometry(const SoupDev &soup, uint32_t n_emissive, uint32_t *emissive_
         k_wide_emit<<<blocks(W, 256), 256, 0, s>>>(words, wkids, rank, wide_of_bin, W,
                                                    reinterpret_cast<float *>(out->geom + out->wide_base));
     }
-    if (emissive_slots && n_emissive > 0 && soup.light_of)
+    uint32_t *records = nullptr;
+    if (found > 0) {
+        if (sc.get(4ull * found, &records, "BVH build: light records")) {
+            err = "hip: device BVH build: light records";
+            al.release(out->geom, out->geom_vec4 * 16);
+            al.release(out->attrs, out->attrs_vec4 * 16);
+            out->geom = out->attrs = nullptr;
+            return OXPT_E_BACKEND;
+        }
+        k_emissive_slots<<<blocks(N, 256), 256, 0, s>>>(ids, sd.light_of, N, found, found_slots);
+        k_emissive_records<<<1, 64, 0, s>>>(wgt_c, tri_of, found_slots, found, records);
+    } else if (emissive_slots && n_emissive > 0 && soup.light_of) {
         k_emissive_slots<<<blocks(N, 256), 256, 0, s>>>(ids, soup.light_of, N, n_emissive, emissive_slots);
+    }
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    out->n_emissive = found;
+    out->emissive_records.assign(4ull * found, 0u);
+    if (e == hipSuccess && found > 0)
+        e = hipMemcpy(out->emissive_records.data(), records, 16ull * found, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return fail_out("emit", e);
     out->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;

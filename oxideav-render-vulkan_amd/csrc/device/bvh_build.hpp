@@ -27,6 +27,9 @@ struct SoupDev {
     const uint32_t *material = nullptr;
     const uint8_t *flags = nullptr;   // OXPT_F_* per triangle
     const uint32_t *light_of = nullptr;  // emissive-light index or OXPT_NONE
+    // Lights found on the device instead: per-material emission luminance.
+    const float *mat_lum = nullptr;
+    uint32_t n_mat = 0;
 };
 
 // Device allocation through the caller (the context's budget accounting).
@@ -43,6 +46,9 @@ struct BuiltGeometry {
     uint32_t node_count = 0, tri_base = 0, wide_base = 0;
     uint32_t bvh_depth = 0, wide_depth = 0;
     double ms = 0.0;  // wall time of the build, upload of the soup excluded
+    // Lights found on the device: their records {cdf, pmf, triangle, slot}.
+    uint32_t n_emissive = 0;
+    std::vector<uint32_t> emissive_records;
 };
 
 // Subtrees of at most this many triangles are finished by one wave in LDS.
@@ -52,7 +58,9 @@ constexpr uint32_t kBuildMaxTris = 1u << 24;
 
 // Builds everything on stream `s`. `emissive_slots` (n_emissive words,
 // device memory, may be null) receives the leaf slot of every emissive
-// triangle, indexed by its light index. Returns 0 or a negative OXPT_E_*
+// triangle, indexed by its light index. With soup.mat_lum set the lights
+// are found here instead (n_emissive, emissive_slots ignored) and their
+// records returned in out->emissive_records. Returns 0 or a negative OXPT_E_*
 // code with `err` set; temporaries are released either way.
 int build_geometry(const SoupDev &soup, uint32_t n_emissive, uint32_t *emissive_slots, hipStream_t s,
                    const DevAlloc &a, BuiltGeometry *out, std::string &err);
@@ -72,5 +80,11 @@ struct AtlasChainDev {
 // atlas and match the level sizes (checked by the caller).
 int build_atlas(const std::vector<AtlasChainDev> &chains, const float *lut, uint16_t *atlas, uint32_t side,
                 uint32_t layers, hipStream_t s, const DevAlloc &a, std::string &err);
+
+// Environment section (marginal h + 1, conditional h x (w + 1), mass w h,
+// radiance 3 w h words) into `out` on stream `s`; rgb (w h x 3) and row_w
+// (h row weights sin(pi (y + 0.5) / h)) in device memory.
+int build_env(const float *rgb, uint32_t w, uint32_t h, float intensity, const double *row_w, uint32_t *out,
+              hipStream_t s, const DevAlloc &a, std::string &err);
 
 }  // namespace oxd

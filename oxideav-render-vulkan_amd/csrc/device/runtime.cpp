@@ -624,6 +624,8 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
     if (d->n_emissive > 0 && (!d->light_of || (size_t)d->elight_offset + 4ull * d->n_emissive > d->table_words))
         return fail(c, OXPT_E_ARG, "scene build: emissive records need light_of and fit the tables");
     if (d->atlas && d->atlas_build) return fail(c, OXPT_E_ARG, "scene build: atlas texels or atlas_build, not both");
+    if (d->material_emission && (d->light_of || d->n_emissive || d->n_materials == 0))
+        return fail(c, OXPT_E_ARG, "scene build: lights are either found on the device or given, not both");
     const size_t n = d->n_tris;
     uint32_t any = 0;
     for (size_t i = 0; i < n; i++) any |= d->flags[i];
@@ -648,11 +650,11 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
     if (setdev(c)) return OXPT_E_BACKEND;
     const auto t0 = std::chrono::steady_clock::now();
     // The soup and the emissive slot words live on the device for the build only.
-    void *dev[10] = {};
-    size_t dev_bytes[10] = {};
+    void *dev[11] = {};
+    size_t dev_bytes[11] = {};
     auto release = [&]() {
         (void)hipStreamSynchronize(c->stream);
-        for (int i = 0; i < 10; i++) dfree(c, dev[i], dev_bytes[i]);
+        for (int i = 0; i < 11; i++) dfree(c, dev[i], dev_bytes[i]);
     };
     int rc = OXPT_OK;
     for (int i = 0; i < 9 && rc == OXPT_OK; i++) {
@@ -666,6 +668,14 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
     if (rc == OXPT_OK && d->n_emissive > 0) {
         dev_bytes[9] = 4ull * d->n_emissive;
         rc = dalloc(c, dev_bytes[9], &dev[9], "scene build: emissive slots");
+    }
+    if (rc == OXPT_OK && d->material_emission) {
+        dev_bytes[10] = 4ull * d->n_materials;
+        if ((rc = dalloc(c, dev_bytes[10], &dev[10], "scene build: material emission")) == OXPT_OK) {
+            hipError_t e = hipMemcpyAsync(dev[10], d->material_emission, dev_bytes[10], hipMemcpyHostToDevice,
+                                          c->stream);
+            if (e != hipSuccess) rc = hip_fail(c, "scene build: material emission upload", e);
+        }
     }
     if (rc == OXPT_OK) {
         hipError_t e = hipStreamSynchronize(c->stream);
@@ -687,6 +697,8 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
     sd.material = (const uint32_t *)dev[6];
     sd.flags = (const uint8_t *)dev[7];
     sd.light_of = (const uint32_t *)dev[8];
+    sd.mat_lum = (const float *)dev[10];
+    sd.n_mat = d->material_emission ? d->n_materials : 0u;
     oxd::DevAlloc al;
     al.alloc = [c](size_t bytes, void **p, const char *what) { return dalloc(c, bytes, p, what); };
     al.release = [c](void *p, size_t bytes) { dfree(c, p, bytes); };
@@ -699,6 +711,9 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
     }
     // Tables with the emissive slot words filled in, then the atlas.
     std::vector<uint32_t> tables(d->tables, d->tables + d->table_words);
+    // Lights found on the device: their records follow the tables.
+    const uint32_t elight_offset = d->material_emission ? (uint32_t)d->table_words : d->elight_offset;
+    tables.insert(tables.end(), bg.emissive_records.begin(), bg.emissive_records.end());
     std::vector<uint32_t> slots(d->n_emissive);
     if (d->n_emissive > 0) {
         hipError_t e = hipMemcpy(slots.data(), dev[9], 4ull * d->n_emissive, hipMemcpyDeviceToHost);
@@ -756,6 +771,8 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
         info->upload_ms = upload_ms;
         info->build_ms = bg.ms;
         info->atlas_ms = atlas_ms;
+        info->n_emissive = d->material_emission ? bg.n_emissive : d->n_emissive;
+        info->elight_offset = elight_offset;
     }
     *out = s;
     return OXPT_OK;
@@ -798,6 +815,50 @@ int oxpt_scene_set_tables(oxpt_ctx *c, oxpt_scene *s, const uint32_t *tables, si
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) s->host_tables.assign(tables, tables + words);
     return e == hipSuccess ? OXPT_OK : hip_fail(c, "tables update", e);
+}
+
+int oxpt_scene_set_tables_env(oxpt_ctx *c, oxpt_scene *s, const uint32_t *tables, size_t words,
+                              const oxpt_env_desc *env) {
+    if (!c || !s || !tables || words == 0 || !env || s->ctx != c) return OXPT_E_ARG;
+    if (!env->rgb || !env->row_weight || env->w == 0 || env->h == 0 || (uint64_t)env->w * env->h > (1ull << 28))
+        return fail(c, OXPT_E_ARG, "environment: rgb, row weights and 1 .. 2^28 texels are required");
+    const size_t w = env->w, h = env->h, n = w * h;
+    const size_t env_words = (h + 1) + h * (w + 1) + n + 3 * n;
+    const size_t total = words + env_words;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, "tables update", e);
+    if (total != s->table_words) {
+        void *p = nullptr;
+        if (int rc = dalloc(c, total * 4, &p, "scene tables")) return rc;
+        dfree(c, s->tables, s->table_words * 4);
+        s->bytes = s->bytes - s->table_words * 4 + total * 4;
+        s->tables = (uint32_t *)p;
+        s->table_words = total;
+    }
+    void *rgb = nullptr, *roww = nullptr;
+    int rc = dalloc(c, n * 12, &rgb, "environment: texels");
+    if (rc == OXPT_OK) rc = dalloc(c, h * 8, &roww, "environment: row weights");
+    if (rc == OXPT_OK) {
+        e = hipMemcpyAsync(s->tables, tables, words * 4, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(rgb, env->rgb, n * 12, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(roww, env->row_weight, h * 8, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, "environment upload", e);
+    }
+    if (rc == OXPT_OK) {
+        oxd::DevAlloc al;
+        al.alloc = [c](size_t bytes, void **p, const char *what) { return dalloc(c, bytes, p, what); };
+        al.release = [c](void *p, size_t bytes) { dfree(c, p, bytes); };
+        std::string err;
+        if ((rc = oxd::build_env((const float *)rgb, env->w, env->h, env->intensity, (const double *)roww,
+                                 s->tables + words, c->stream, al, err)))
+            fail(c, rc, err);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    dfree(c, rgb, n * 12);
+    dfree(c, roww, h * 8);
+    if (rc == OXPT_OK) s->host_tables.assign(tables, tables + words);
+    return rc;
 }
 
 uint64_t oxpt_scene_bytes(const oxpt_scene *s) { return s ? s->bytes : 0; }

@@ -220,6 +220,16 @@ void sheen_table(float out[OXPT_SHEEN_WORDS]) {
     std::memcpy(out, lut.data(), OXPT_SHEEN_WORDS * sizeof(float));
 }
 
+std::vector<float> material_emission(const HostScene &s) {
+    std::vector<float> out(s.materials.size(), 0.0f);
+    for (size_t i = 0; i < s.materials.size(); i++) {
+        const oxpt_material &m = s.materials[i];
+        bool emits = !m.unlit && (m.emissive[0] > 0.0f || m.emissive[1] > 0.0f || m.emissive[2] > 0.0f);
+        out[i] = emits ? luminance(m.emissive) : 0.0f;
+    }
+    return out;
+}
+
 void texel_decode_lut(float out[512]) {
     for (int i = 0; i < 256; i++) {
         out[i] = srgb_to_linear((float)i / 255.0f);
@@ -323,16 +333,20 @@ CameraFrame resolve_camera(const CameraSource &s, uint32_t w, uint32_t h) {
     return f;
 }
 
-Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *light_of_out, bool atlas) {
+Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *light_of_out, bool atlas,
+                  bool emissive) {
     Packed pk;
     const Soup &soup = s.soup;
     size_t ntri = soup.count();
+    // Without `emissive` the lights are found by the device builder, which
+    // appends their records to the tables.
+    const size_t n_scan = emissive ? ntri : 0;
     // ---- Emissive lights (EmissiveLights::build): every triangle whose
     // material emits (not unlit, non-black factor) with positive area;
     // selection weight = area x luminance(emissive factor).
     std::vector<uint32_t> light_of(ntri, OXPT_NONE), elight_tris;
     std::vector<double> weights;
-    for (size_t g = 0; g < ntri; g++) {
+    for (size_t g = 0; g < n_scan; g++) {
         const oxpt_material &m = s.materials.at(soup.material[g]);
         bool emits = !m.unlit && (m.emissive[0] > 0.0f || m.emissive[1] > 0.0f || m.emissive[2] > 0.0f);
         if (!emits) continue;
@@ -569,15 +583,36 @@ Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *ligh
     return pk;
 }
 
+EnvWords env_layout(uint32_t w, uint32_t h) {
+    EnvWords out;
+    out.w = std::max(w, 1u);
+    out.h = std::max(h, 1u);
+    out.marg = 0;
+    out.cond = out.h + 1;
+    out.mass = out.cond + out.h * (out.w + 1);
+    out.rad = out.mass + out.w * out.h;
+    return out;
+}
+
+std::vector<double> env_row_weights(uint32_t h) {
+    const double PI = 3.14159265358979323846;
+    h = std::max(h, 1u);
+    std::vector<double> st(h);
+    for (uint32_t y = 0; y < h; y++) st[y] = std::sin(PI * (y + 0.5) / h);
+    return st;
+}
+
 void bind(const Packed &st, const EnvMap *env, const CameraFrame &cam, const oxpt_render_options &o, uint32_t w,
-          uint32_t h, std::vector<uint32_t> &tables, oxpt_params &P) {
+          uint32_t h, std::vector<uint32_t> &tables, oxpt_params &P, bool env_on_host) {
     tables = st.tables;
     uint32_t env_base = (uint32_t)tables.size();
     EnvWords ew;
     bool has_env = env != nullptr;
-    if (has_env) {
+    if (has_env && env_on_host) {
         ew = env_words(*env);
         tables.insert(tables.end(), ew.words.begin(), ew.words.end());
+    } else if (has_env) {
+        ew = env_layout(env->w, env->h);  // the words follow `tables` on the device
     }
     if (tables.empty()) tables.push_back(0);
     // P_env: 0.5 with both an environment and emissive triangles, 1 with
@@ -632,11 +667,11 @@ EnvWords env_words(const EnvMap &e) {
     uint32_t w = std::max(e.w, 1u), h = std::max(e.h, 1u);
     out.w = w;
     out.h = h;
-    const double PI = 3.14159265358979323846;
+    const std::vector<double> row_w = env_row_weights(h);
     std::vector<double> wt((size_t)w * h, 0.0), rows(h, 0.0);
     std::vector<float> rad((size_t)w * h * 3, 0.0f);
     for (uint32_t y = 0; y < h; y++) {
-        double st = std::sin(PI * (y + 0.5) / h);
+        double st = row_w[y];
         for (uint32_t x = 0; x < w; x++) {
             size_t i = (size_t)y * w + x;
             float c[3] = {0, 0, 0};

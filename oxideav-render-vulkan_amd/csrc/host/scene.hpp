@@ -110,15 +110,28 @@ uint32_t build_wide(const std::vector<uint32_t> &node_words, std::vector<float> 
 // With atlas = false the texels are left to the device builder
 // (oxpt_atlas_desc): descriptors and placement are written, pk.atlas stays
 // empty.
+// With emissive = false no emissive records are written (n_emissive = 0):
+// the device builder finds the lights from material_emission() and appends
+// the records at the end of the tables.
 Packed pack_scene(const HostScene &s, bool geometry = true, std::vector<uint32_t> *light_of_out = nullptr,
-                  bool atlas = true);
+                  bool atlas = true, bool emissive = true);
+// Per material: luminance of the emissive factor for emitting materials
+// (not unlit, some component > 0), else 0 (EmissiveLights::build's weight
+// per unit area).
+std::vector<float> material_emission(const HostScene &s);
 // Environment section (env_words, src/pathtrace.rs:1428-1461).
 EnvWords env_words(const EnvMap &e);
 
 // Tables with the environment appended and the 176-byte Params
 // (ensure_binding, src/pathtrace.rs:847-926).
+// env_on_host = false: Params get the environment layout but its words
+// are not appended (oxpt_scene_set_tables_env builds them on the device).
 void bind(const Packed &pk, const EnvMap *env, const CameraFrame &cam, const oxpt_render_options &o, uint32_t w,
-          uint32_t h, std::vector<uint32_t> &tables, oxpt_params &P);
+          uint32_t h, std::vector<uint32_t> &tables, oxpt_params &P, bool env_on_host = true);
+// Offsets of the environment section (no words), and the per-row weights
+// sin(pi (y + 0.5) / h) its texel weights use.
+EnvWords env_layout(uint32_t w, uint32_t h);
+std::vector<double> env_row_weights(uint32_t h);
 
 void sobol_directions(uint32_t out[OXPT_SOBOL_WORDS]);
 void sheen_table(float out[OXPT_SHEEN_WORDS]);

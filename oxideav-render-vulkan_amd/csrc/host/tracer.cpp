@@ -69,13 +69,15 @@ struct oxpt_tracer {
     oxpt_build_info build_info{};
 
     int upload_device(const oxpt::HostScene &s) {
-        std::vector<uint32_t> light_of;
-        auto pk = std::make_unique<oxpt::Packed>(oxpt::pack_scene(s, false, &light_of, false));
+        // Host: materials, lights, texture placement, Sobol' / sheen tables.
+        // Device: lights, BVH, triangle / attribute words, 4-wide nodes, atlas.
+        auto pk = std::make_unique<oxpt::Packed>(oxpt::pack_scene(s, false, nullptr, false, false));
         if (dscene) {
             oxpt_scene_destroy(dscene);
             dscene = nullptr;
         }
         const oxpt::Soup &so = s.soup;
+        const std::vector<float> mat_lum = oxpt::material_emission(s);
         oxpt_soup_desc d{};
         d.n_tris = so.count();
         d.pos = so.pos.data();
@@ -86,14 +88,12 @@ struct oxpt_tracer {
         d.colors = so.colors.empty() ? nullptr : so.colors.data();
         d.material = so.material.data();
         d.flags = so.flags.data();
-        d.light_of = light_of.data();
-        d.n_emissive = pk->n_emissive;
-        d.elight_offset = pk->off_elights;
         d.tables = pk->tables.data();
         d.table_words = pk->tables.size();
-        d.atlas = nullptr;
         d.atlas_side = pk->atlas_side;
         d.atlas_layers = pk->atlas_layers;
+        d.material_emission = mat_lum.data();
+        d.n_materials = (uint32_t)mat_lum.size();
         // Texels: decoded, filtered and converted on the device.
         std::vector<oxpt_texture_desc> tex(s.textures.size());
         for (size_t i = 0; i < s.textures.size(); i++)
@@ -102,10 +102,18 @@ struct oxpt_tracer {
         oxpt::texel_decode_lut(lut);
         oxpt_atlas_desc ad{tex.size(), tex.data(), pk->chains.size(), pk->chains.data(), lut};
         d.atlas_build = pk->chains.empty() ? nullptr : &ad;
-        std::vector<uint32_t> slots(pk->n_emissive);
-        int rc = oxpt_scene_build(ctx, &d, slots.data(), &dscene, &build_info);
+        int rc = oxpt_scene_build(ctx, &d, nullptr, &dscene, &build_info);
         if (rc) return dev_fail(rc);
-        for (uint32_t i = 0; i < pk->n_emissive; i++) pk->tables[pk->off_elights + 4 * i + 3] = slots[i];
+        // The tables now end with the emissive records found on the device.
+        size_t tw = 0;
+        rc = oxpt_scene_read(ctx, dscene, 2, nullptr, 0, &tw);
+        if (!rc) {
+            pk->tables.resize(tw);
+            rc = oxpt_scene_read(ctx, dscene, 2, pk->tables.data(), tw * 4, nullptr);
+        }
+        if (rc) return dev_fail(rc);
+        pk->n_emissive = build_info.n_emissive;
+        pk->off_elights = build_info.elight_offset;
         pk->node_count = build_info.node_count;
         pk->tri_base = build_info.tri_base;
         pk->wide_base = build_info.wide_base;
@@ -166,10 +174,38 @@ struct oxpt_tracer {
     // Tables (scene + environment) and Params (ensure_binding).
     int ensure_binding() {
         if (binding) return OXPT_OK;
-        oxpt::bind(*packed, env ? &*env : nullptr, cam, opts, w, h, tables, params);
-        int rc = oxpt_scene_set_tables(ctx, dscene, tables.data(), tables.size());
+        const bool env_dev = device_build && env && words_on_device_tables_ok();
+        oxpt::bind(*packed, env ? &*env : nullptr, cam, opts, w, h, tables, params, !env_dev);
+        int rc;
+        if (env_dev) {
+            // Environment sampling tables built on the device after the scene tables.
+            const std::vector<double> row_w = oxpt::env_row_weights(env->h);
+            oxpt_env_desc ed{env->w, env->h, env->rgb.data(), env->intensity, 0u, row_w.data()};
+            rc = oxpt_scene_set_tables_env(ctx, dscene, tables.data(), tables.size(), &ed);
+            tables_on_device = true;
+        } else {
+            rc = oxpt_scene_set_tables(ctx, dscene, tables.data(), tables.size());
+            tables_on_device = false;
+        }
         if (rc) return dev_fail(rc);
         binding = true;
+        return OXPT_OK;
+    }
+    // The device environment builder needs a full w x h x 3 texel array.
+    bool words_on_device_tables_ok() const {
+        return env && env->w > 0 && env->h > 0 && env->rgb.size() == (size_t)env->w * env->h * 3;
+    }
+    bool tables_on_device = false;  // host `tables` lacks the environment words
+    int fetch_tables() {
+        if (!tables_on_device) return OXPT_OK;
+        size_t tw = 0;
+        int rc = oxpt_scene_read(ctx, dscene, 2, nullptr, 0, &tw);
+        if (!rc) {
+            tables.resize(tw);
+            rc = oxpt_scene_read(ctx, dscene, 2, tables.data(), tw * 4, nullptr);
+        }
+        if (rc) return dev_fail(rc);
+        tables_on_device = false;
         return OXPT_OK;
     }
     int sync(const oxpt_host_scene *scene, const oxpt_render_options &o, int *reset_out) {
@@ -439,6 +475,8 @@ int oxpt_tracer_words(oxpt_tracer *t, int which, const void **ptr, size_t *count
     if (int rc = t->ensure_binding()) return rc;
     if (which <= 1 || which == 3)
         if (int rc = t->fetch_words()) return rc;
+    if (which == 2)
+        if (int rc = t->fetch_tables()) return rc;
     switch (which) {
         case 0: *ptr = t->packed->geom.data(); *count = t->packed->geom.size() / 4; return OXPT_OK;
         case 1: *ptr = t->packed->attrs.data(); *count = t->packed->attrs.size() / 4; return OXPT_OK;

@@ -88,14 +88,14 @@ class SoupDesc(C.Structure):
         ("uv1", P(f32)), ("colors", P(f32)), ("material", P(u32)), ("flags", P(C.c_uint8)),
         ("light_of", P(u32)), ("n_emissive", u32), ("elight_offset", u32), ("tables", P(u32)),
         ("table_words", sz), ("atlas", P(C.c_uint16)), ("atlas_side", u32), ("atlas_layers", u32),
-        ("atlas_build", C.c_void_p),
+        ("atlas_build", C.c_void_p), ("material_emission", P(f32)), ("n_materials", u32),
     ]
 
 
 class BuildInfo(C.Structure):
     _fields_ = [("node_count", u32), ("tri_base", u32), ("wide_base", u32), ("bvh_depth", u32),
                 ("wide_depth", u32), ("pad", u32), ("upload_ms", C.c_double), ("build_ms", C.c_double),
-                ("atlas_ms", C.c_double)]
+                ("atlas_ms", C.c_double), ("n_emissive", u32), ("elight_offset", u32)]
 
 
 class TexBinding(C.Structure):
@@ -194,6 +194,7 @@ _sig("oxpt_ctx_set_budget", C.c_int, _vp, u64)
 _sig("oxpt_ctx_bytes_in_use", u64, _vp)
 _sig("oxpt_scene_upload", C.c_int, _vp, P(SceneDesc), P(_vp))
 _sig("oxpt_scene_set_tables", C.c_int, _vp, _vp, P(u32), sz)
+_sig("oxpt_scene_set_tables_env", C.c_int, _vp, _vp, P(u32), sz, _vp)
 _sig("oxpt_scene_destroy", None, _vp)
 _sig("oxpt_scene_build", C.c_int, _vp, P(SoupDesc), P(u32), P(_vp), P(BuildInfo))
 _sig("oxpt_scene_read", C.c_int, _vp, _vp, C.c_int, _vp, sz, P(sz))

@@ -81,7 +81,25 @@ def textures_odd():
     return s
 
 
+def emissive_many():
+    """Hundreds of emissive triangles of uneven area, some of zero area,
+    an unlit emitter, a material with a negative emission channel and a
+    black one: the lights, their order and their CDF are found on the
+    device."""
+    rng = np.random.default_rng(3)
+    s = oxpt.Scene()
+    mats = [s.add_material(oxpt.material(base_color=(0.5, 0.5, 0.5, 1.0), emissive=tuple(e), unlit=u))
+            for e, u in (((4.0, 3.0, 2.0), 0), ((0.0, 0.0, 0.0), 0), ((2.0, 2.0, 2.0), 1), ((3.0, -1.0, 0.5), 0),
+                         ((0.0, 0.0, 7.0), 0))]
+    n = 900
+    pos = rng.uniform(-2, 2, (n, 9)).astype(np.float32)
+    pos[::37, 6:9] = pos[::37, 3:6]  # degenerate: zero area
+    s.add_triangles(pos, rng.integers(0, len(mats), n).astype(np.uint32))
+    return s
+
+
 SCENES = {
+    "emissive_many": emissive_many,
     "textures_odd": textures_odd,
     "checker": ptscenes.checker_floor,
     "alpha": ptscenes.alpha_planes,
@@ -164,3 +182,30 @@ def test_device_build_switch_and_empty_scene():
     pt.refine(1)
     img = pt.image()
     assert (img[..., :3] == np.array(OPTS.background[:3], np.uint8)).all()
+
+
+def env_maps():
+    rng = np.random.default_rng(9)
+    noisy = rng.uniform(-0.5, 3.0, (37, 64, 3)).astype(np.float32)
+    noisy[3, 5] = (np.nan, 1.0, 1.0)
+    noisy[7, 9, 1] = np.inf
+    noisy[11] = 0.0  # a black row
+    return {
+        "sky": oxpt.EnvironmentMap.sky(256, 128, 2),
+        "noisy": oxpt.EnvironmentMap(noisy, intensity=1.7),
+        "black": oxpt.EnvironmentMap(np.zeros((8, 16, 3), np.float32)),
+    }
+
+
+@pytest.mark.parametrize("env_name", ["sky", "noisy", "black"])
+def test_device_environment_tables_equal_the_host(env_name):
+    env = env_maps()[env_name]
+    scene = oxpt.Scene.room(0)
+    pk = oxpt.Packed(scene, OPTS, env)
+    pt = device_tracer(scene, OPTS, env)
+    assert bytes(pt.params()) == bytes(pk.params)
+    dev = pt.scene_words(2)
+    assert dev.size == pk.tables.size
+    bad = np.flatnonzero(dev != pk.tables)
+    assert bad.size == 0, f"{env_name}: tables differ at {bad[:8]} ({bad.size} words)"
+    assert np.array_equal(pt.words(2), pk.tables)
