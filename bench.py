@@ -284,12 +284,61 @@ def bench_build(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_display(args):
+    """The progressive viewer's per-frame cost at 1920x1080 on the room
+    scene (the reference's Cornell 1080p rows): refine(1) + draw (resolve
+    into device memory, draw_texture) vs refine(1) + image() (resolve +
+    readback), and each resolve alone. Median of --steps frames."""
+    import oxpt
+
+    scene = oxpt.Scene.room(0)
+    opts = oxpt.RenderOptions(width=1920, height=1080, samples_per_pixel=1 << 20, max_bounces=8, rr_start=3)
+    pt = oxpt.GpuPathTracer(0)
+    pt.sync(scene, opts)
+    target = oxpt.DeviceBuffer(1920 * 1080 * 4)
+
+    def draw():
+        pt.draw(target.ptr, target.nbytes)
+        pt.finish()
+
+    def image():
+        pt.image()
+
+    def refine_draw():
+        pt.refine(1)
+        draw()
+
+    def refine_image():
+        pt.refine(1)
+        image()
+
+    res = {}
+    for name, fn in (("refine1_draw", refine_draw), ("refine1_image", refine_image), ("draw", draw), ("image", image)):
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        times = []
+        for _ in range(max(args.steps, 5) * 4):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+        res[name] = round(sorted(times)[len(times) // 2] * 1e3, 3)
+        log(f"{name}: {res[name]} ms")
+    same = bool(np.array_equal(target.read().reshape(1080, 1920, 4), pt.image()))
+    out = {"metric": "progressive frame ms at 1920x1080: refine(1) + draw (device resolve)", "value": res["refine1_draw"],
+           "unit": "ms", "n_gpus": 1, "steps": max(args.steps, 5) * 4, "warmup": max(args.warmup, 1),
+           "ms_per_step": res["refine1_draw"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (seeded procedural scene generators)",
+           "config": {"workload": "viewer loop: closed room (36 tris), 1920x1080, 1 spp per frame, depth 8"},
+           "frame_ms": res, "draw_equals_image": same}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4", "build"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4", "build", "display"])
     ap.add_argument("--build-scenes", default="c1,c2", help="build: scenes to prepare (c0, c1, c2)")
     ap.add_argument("--rate", type=float, default=50.0, help="c4: Poisson arrival rate (req/s)")
     ap.add_argument("--duration", type=float, default=60.0, help="c4: arrival window (s)")
@@ -332,6 +381,8 @@ def main():
         return bench_serving(args)
     if args.workload == "build":
         return bench_build(args)
+    if args.workload == "display":
+        return bench_display(args)
     scene, env, opts, desc = workload(args.workload)
     pt = oxpt.GpuPathTracer(0)
     log("device:", pt.adapter_summary())

@@ -217,6 +217,14 @@ int oxpt_finish(oxpt_ctx *ctx);
  * src/shaders/resolve.wgsl:151-188, src/gpu.rs:1060-1117). */
 int oxpt_resolve(oxpt_ctx *ctx, const oxpt_accum *acc, const oxpt_resolve_params *rp,
                  void *host_out, size_t host_bytes);
+/* Resolve into caller-owned device memory of the context's GPU, tightly
+ * packed rows (RGBA8 or RGBA32F as oxpt_resolve), stream-ordered and
+ * without blocking or readback: draw_texture, which encodes the resolve
+ * pass into the caller's texture view (src/pathtrace.rs:687-691). A
+ * viewer maps that memory for display (HIP / graphics interop); oxpt_finish
+ * orders host reads after it. Host pointers are refused (OXPT_E_ARG). */
+int oxpt_resolve_device(oxpt_ctx *ctx, const oxpt_accum *acc, const oxpt_resolve_params *rp,
+                        void *device_out, size_t bytes);
 
 /* Kernel timing: HIP events around every trace launch on the context's
  * stream. Returns launches and total milliseconds since the last reset. */
@@ -426,6 +434,10 @@ int oxpt_tracer_refine(oxpt_tracer *t, uint32_t samples);
 int oxpt_tracer_finish(oxpt_tracer *t);
 int oxpt_tracer_image(oxpt_tracer *t, uint8_t *out_rgba8, size_t bytes);
 int oxpt_tracer_hdr(oxpt_tracer *t, float *out_rgba32f, size_t bytes);
+/* draw_texture (src/pathtrace.rs:687-691): resolve the current accumulation
+ * with the current options into device memory (oxpt_resolve_device): RGBA8
+ * (hdr = 0, w*h*4 bytes) or RGBA32F (hdr = 1); asynchronous. */
+int oxpt_tracer_draw(oxpt_tracer *t, void *device_out, size_t bytes, int hdr);
 void oxpt_tracer_reset(oxpt_tracer *t);
 uint32_t oxpt_tracer_samples(const oxpt_tracer *t);
 int oxpt_tracer_is_converged(const oxpt_tracer *t);

@@ -1350,6 +1350,27 @@ int oxpt_resolve(oxpt_ctx *c, const oxpt_accum *a, const oxpt_resolve_params *rp
     return e == hipSuccess ? OXPT_OK : hip_fail(c, "resolve", e);
 }
 
+int oxpt_resolve_device(oxpt_ctx *c, const oxpt_accum *a, const oxpt_resolve_params *rp, void *dev, size_t bytes) {
+    if (!c || !a || !rp || !dev || a->ctx != c) return OXPT_E_ARG;
+    const size_t need = (size_t)a->w * a->h * (rp->cfg[3] == 1 ? 16 : 4);
+    if (bytes < need) return fail(c, OXPT_E_ARG, "resolve: output buffer too small");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    // Only device memory of this context's GPU (or managed memory) may be
+    // written by the kernel; anything else is refused here.
+    hipPointerAttribute_t at;
+    std::memset(&at, 0, sizeof at);
+    hipError_t e = hipPointerGetAttributes(&at, dev);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, OXPT_E_ARG, "resolve: output is not device memory");
+    }
+    const bool device_mem = at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+    if (!device_mem || (at.type == hipMemoryTypeDevice && at.device != c->device))
+        return fail(c, OXPT_E_ARG, "resolve: output is not device memory of this context's GPU");
+    e = oxd::launch_resolve(a->d, a->w * a->h, *rp, dev, c->stream);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "resolve", e);
+}
+
 int oxpt_debug_pattern_words(oxpt_ctx *c, const oxpt_scene *s, uint32_t seed, size_t n, const uint32_t *x,
                              const uint32_t *y, const uint32_t *index, const uint32_t *p, uint32_t *out4) {
     if (!c || !s || !x || !y || !index || !p || !out4 || s->ctx != c) return OXPT_E_ARG;

@@ -245,7 +245,7 @@ struct oxpt_tracer {
         samples += n;
         return OXPT_OK;
     }
-    int resolve(bool hdr, void *out, size_t bytes) {
+    int resolve(bool hdr, void *out, size_t bytes, bool device = false) {
         if (!accum) {
             // Never synced: an all-background frame of the current size.
             uint32_t ww = std::max(w, 1u), hh = std::max(h, 1u);
@@ -262,7 +262,7 @@ struct oxpt_tracer {
         rp.exposure[1] = (float)samples;
         rp.exposure[2] = (float)w;
         for (int k = 0; k < 4; k++) rp.background[k] = (float)opts.background[k] / 255.0f;
-        int rc = oxpt_resolve(ctx, accum, &rp, out, bytes);
+        int rc = device ? oxpt_resolve_device(ctx, accum, &rp, out, bytes) : oxpt_resolve(ctx, accum, &rp, out, bytes);
         return rc ? dev_fail(rc) : OXPT_OK;
     }
     ~oxpt_tracer() {
@@ -420,6 +420,9 @@ int oxpt_tracer_finish(oxpt_tracer *t) {
 }
 int oxpt_tracer_image(oxpt_tracer *t, uint8_t *out, size_t bytes) { return t ? t->resolve(false, out, bytes) : OXPT_E_ARG; }
 int oxpt_tracer_hdr(oxpt_tracer *t, float *out, size_t bytes) { return t ? t->resolve(true, out, bytes) : OXPT_E_ARG; }
+int oxpt_tracer_draw(oxpt_tracer *t, void *device_out, size_t bytes, int hdr) {
+    return t ? t->resolve(hdr != 0, device_out, bytes, true) : OXPT_E_ARG;
+}
 void oxpt_tracer_reset(oxpt_tracer *t) {
     if (t) t->reset();
 }

@@ -207,6 +207,7 @@ _sig("oxpt_accum_write", C.c_int, _vp, _vp, P(f32), sz)
 _sig("oxpt_refine", C.c_int, _vp, _vp, _vp, P(Params), u32, u32)
 _sig("oxpt_finish", C.c_int, _vp)
 _sig("oxpt_resolve", C.c_int, _vp, _vp, P(ResolveParams), _vp, sz)
+_sig("oxpt_resolve_device", C.c_int, _vp, _vp, P(ResolveParams), _vp, sz)
 _sig("oxpt_ctx_timing", C.c_int, _vp, C.c_int, P(u64), P(C.c_double))
 _sig("oxpt_refine_batch", C.c_int, _vp, P(BatchItem), sz)
 _sig("oxpt_ctx_set_tuning", C.c_int, _vp, P(Tuning))
@@ -244,6 +245,7 @@ _sig("oxpt_tracer_refine", C.c_int, _vp, u32)
 _sig("oxpt_tracer_finish", C.c_int, _vp)
 _sig("oxpt_tracer_image", C.c_int, _vp, P(C.c_uint8), sz)
 _sig("oxpt_tracer_hdr", C.c_int, _vp, P(f32), sz)
+_sig("oxpt_tracer_draw", C.c_int, _vp, _vp, sz, C.c_int)
 _sig("oxpt_tracer_reset", None, _vp)
 _sig("oxpt_tracer_samples", u32, _vp)
 _sig("oxpt_tracer_is_converged", C.c_int, _vp)
@@ -596,6 +598,11 @@ class GpuPathTracer:
         self._check(lib.oxpt_tracer_atlas_shape(self.h, C.byref(s), C.byref(l)), "atlas_shape")
         return s.value, l.value
 
+    def draw(self, device_ptr: int, nbytes: int, hdr: bool = False) -> None:
+        """draw_texture: resolve into device memory at device_ptr (RGBA8 or
+        RGBA32F rows), asynchronously; finish() orders host reads after it."""
+        self._check(lib.oxpt_tracer_draw(self.h, _vp(device_ptr), nbytes, 1 if hdr else 0), "draw")
+
     def set_device_build(self, enable: bool = True) -> None:
         """Prepare scenes on the device (BVH, triangle / attribute words,
         4-wide nodes) from the next sync on; the words are the host packer's."""
@@ -680,6 +687,45 @@ class GpuPathTracer:
         if rc:
             _raise(rc, lib.oxpt_last_error(self.ctx).decode())
         return out
+
+
+_hip = None
+
+
+def _hip_runtime():
+    """The HIP runtime liboxpt.so itself links (already loaded with it)."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMalloc.argtypes = [P(_vp), sz]
+        _hip.hipFree.argtypes = [_vp]
+        _hip.hipMemcpy.argtypes = [_vp, _vp, sz, C.c_int]
+        _hip.hipSetDevice.argtypes = [C.c_int]
+    return _hip
+
+
+class DeviceBuffer:
+    """Raw device memory on one GPU (a draw target for GpuPathTracer.draw),
+    allocated through liboxpt's own HIP runtime."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        hip = _hip_runtime()
+        hip.hipSetDevice(device)
+        p = _vp()
+        if hip.hipMalloc(C.byref(p), nbytes) != 0 or not p.value:
+            raise BackendError(f"hipMalloc of {nbytes} bytes failed")
+        self.ptr, self.nbytes = p.value, nbytes
+
+    def read(self, dtype=np.uint8) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        if _hip_runtime().hipMemcpy(out.ctypes.data_as(_vp), _vp(self.ptr), out.nbytes, 2) != 0:  # device->host
+            raise BackendError("hipMemcpy failed")
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _hip_runtime().hipFree(_vp(self.ptr))
+            self.ptr = None
 
 
 class Device:

@@ -130,3 +130,26 @@ def test_hdr_budget_exceeded_is_backend_error():
     assert oxpt.lib.oxpt_ctx_set_budget(pt.ctx, C.c_uint64(1 << 20)) == 0
     with pytest.raises(oxpt.BackendError):
         pt.sync(oxpt.Scene.terrain(seed=1, gx=64, gz=64, n_spheres=4, subdiv=1, n_emissive=4), opts(64, 64, 1, 1))
+
+
+def test_draw_into_device_memory_equals_the_readback():
+    """draw_texture (src/pathtrace.rs:687-691): the resolve written into
+    device memory without a readback holds the bytes image() / hdr() return;
+    host memory and short buffers are refused."""
+    scene = oxpt.Scene.room(0)
+    opts = oxpt.RenderOptions(width=80, height=48, samples_per_pixel=4, tone_map="reinhard", exposure=1.3)
+    pt = oxpt.GpuPathTracer(0)
+    pt.sync(scene, opts)
+    pt.refine(4)
+    ldr = oxpt.DeviceBuffer(48 * 80 * 4)
+    hdr = oxpt.DeviceBuffer(48 * 80 * 16)
+    pt.draw(ldr.ptr, ldr.nbytes, hdr=False)
+    pt.draw(hdr.ptr, hdr.nbytes, hdr=True)
+    pt.finish()
+    assert np.array_equal(ldr.read(np.uint8).reshape(48, 80, 4), pt.image())
+    assert np.array_equal(hdr.read(np.uint32).reshape(48, 80, 4), pt.hdr().view(np.uint32))
+    host = np.zeros(48 * 80 * 4, np.uint8)
+    with pytest.raises(oxpt.Error):
+        pt.draw(host.ctypes.data, host.size)
+    with pytest.raises(oxpt.Error):
+        pt.draw(ldr.ptr, ldr.nbytes - 1)
