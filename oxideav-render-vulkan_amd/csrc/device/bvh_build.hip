@@ -69,6 +69,7 @@ enum Counter : uint32_t {
     C_WIDE_NEXT,
     C_WIDE_BAD,
     C_WIDE_DEPTH,
+    C_INNER,
     C_COUNT
 };
 
@@ -108,6 +109,17 @@ __device__ inline float box_area(const float lo[3], const float hi[3]) {
     float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
     if (!(dx >= 0.0f && dy >= 0.0f && dz >= 0.0f)) return 0.0f;
     return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+// Element i of a 3-array by selects (a dynamic index would place the array
+// in scratch memory).
+template <typename T>
+__device__ inline T pick3(const T a[3], uint32_t i) {
+    return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
+}
+
+__device__ inline uint32_t pick6(const uint32_t a[6], uint32_t i) {
+    return i < 3 ? pick3(a, i) : pick3(a + 3, i - 3);
 }
 
 __device__ inline float coord(const float4 *tlo, const float4 *thi, const float *tcz, uint32_t t, uint32_t ax) {
@@ -255,7 +267,7 @@ __device__ Decision decide(const uint32_t *sb, const NodeF &f, uint32_t n, uint3
     unsigned long long best = ~0ull;
     for (uint32_t pass = 0; pass < 2; pass++) {
         const uint32_t ax = pass * 2 + (lane >> 5), i = lane & 31;
-        const bool axv = ax < 3 && ok[ax < 3 ? ax : 0];
+        const bool axv = ax < 3 && pick3(ok, ax);
         uint32_t L[7], R[7];  // lo3 hi3 count
         if (ax < 3) {
             const uint32_t *p = &sb[(ax * kBins + i) * kBinWords];
@@ -344,8 +356,8 @@ __device__ void write_children(TNode *nodes, uint32_t parent, const TNode &pn, u
         const uint32_t w = lane & 15, side = lane >> 4;
         if (w < 12 && side <= 1) (side ? R : L)[w] = v;
     } else if (lane < 12) {
-        L[lane] = lane < 6 ? (mbox ? mbox[lane] : identity(lane)) : pn.cb[lane - 6];
-        R[lane] = lane < 6 ? (mbox ? mbox[6 + lane] : identity(lane)) : pn.cb[lane - 6];
+        L[lane] = lane < 6 ? (mbox ? mbox[lane] : identity(lane)) : pick6(pn.cb, lane - 6);
+        R[lane] = lane < 6 ? (mbox ? mbox[6 + lane] : identity(lane)) : pick6(pn.cb, lane - 6);
     }
     const uint32_t depth = (pn.depth & ~kInner) + 1;
     if (lane == 32) {
@@ -451,6 +463,7 @@ __global__ __launch_bounds__(64) void k_split_large(const uint32_t *open, TNode 
     write_children(nodes, v, pn, c, d, sb, nullptr, m, lane);
     if (lane == 0) {
         decs[k] = make_uint4(d.kind | d.axis << 8, d.split, m, c);
+        atomicAdd(&counters[C_INNER], 1u);
         if (d.kind == 2) counters[C_MEDIAN] = 1;
         const uint32_t depth = (pn.depth & ~kInner) + 1;
         atomicMax(&counters[C_DEPTH], depth);
@@ -598,33 +611,45 @@ __global__ __launch_bounds__(kLargeBlock) void k_median_bounds(const uint32_t *o
 
 // ---- 3. small subtrees, one wave each -------------------------------------
 
+// A pending node of the one-wave build: its record in TNode word order
+// (box, centroid box, b, n, parent, depth).
+constexpr uint32_t kRec = 16;
+
 __global__ __launch_bounds__(64) void k_small(const uint32_t *small, TNode *nodes, uint32_t *ids, const float4 *tlo,
                                               const float4 *thi, const float *tcz, uint32_t *counters) {
-    __shared__ uint32_t s_ids[kBuildSmall], s_tmp[kBuildSmall];
-    __shared__ uint32_t sb[kNodeBins];
-    __shared__ uint32_t stk[kStackCap];  // temp node indices
-    __shared__ uint32_t sh[2];
-    __shared__ uint32_t s_mbox[12];
+    __shared__ uint32_t s_ids[kBuildSmall];
+    __shared__ uint32_t sb[kNodeBins];            // bins, then the partition's scratch
+    __shared__ uint32_t stk[kStackCap * kRec];    // pending node records
+    __shared__ uint32_t sh[4];
     const uint32_t lane = threadIdx.x;
     const uint32_t root = small[blockIdx.x];
     const uint32_t b0 = nodes[root].b, n0 = nodes[root].n;
     for (uint32_t i = lane; i < n0; i += 64) s_ids[i] = ids[b0 + i];
-    uint32_t sp = 1, max_depth = 0;
-    if (lane == 0) stk[0] = root;
+    // The subtree's nodes: at most 2 n0 - 2 below the root, reserved at once.
+    if (lane == 0) sh[0] = n0 >= 2 ? atomicAdd(&counters[C_NODES], 2 * n0 - 2) : 0u;
+    if (lane < kRec) stk[lane] = reinterpret_cast<const uint32_t *>(&nodes[root])[lane];
+    __syncthreads();
+    const uint32_t base = sh[0], reserved = n0 >= 2 ? 2 * n0 - 2 : 0u;
+    uint32_t used = 0, splits = 0, max_depth = 0, sp = 1;
+    uint32_t node = root;  // temp index of the record on top of the stack
+    __shared__ uint32_t s_node[kStackCap];
+    if (lane == 0) s_node[0] = root;
     __syncthreads();
     while (sp > 0) {
         sp--;
-        const uint32_t v = stk[sp];
-        // The node record was written by this wave (or the large phase):
-        // read it back through the same wave's view of memory.
-        __threadfence_block();
-        const TNode pn = nodes[v];
-        const uint32_t n = pn.n, depth = pn.depth & ~kInner, rb = pn.b - b0;
+        const uint32_t *rec = &stk[sp * kRec];
+        node = s_node[sp];
+        uint32_t box[6], cb[6];
+        for (int w = 0; w < 6; w++) {
+            box[w] = rec[w];
+            cb[w] = rec[6 + w];
+        }
+        const uint32_t nb = rec[12], n = rec[13], depth = rec[15] & ~kInner, rb = nb - b0;
         max_depth = max(max_depth, depth);
-        __syncthreads();
+        __syncthreads();  // the record is read before anything overwrites its slot
         if (n <= 1 || depth >= kMaxDepth) continue;
         for (uint32_t w = lane; w < kNodeBins; w += 64) sb[w] = identity(w % kBinWords);
-        const NodeF f = decode(pn.box, pn.cb);
+        const NodeF f = decode(box, cb);
         float scale[3];
         bool ok[3];
         for (int a = 0; a < 3; a++) {
@@ -658,21 +683,74 @@ __global__ __launch_bounds__(64) void k_small(const uint32_t *small, TNode *node
         __syncthreads();
         const Decision d = decide(sb, f, n, depth);
         if (d.kind == 0) continue;
+        const uint32_t c = base + used;  // left child; right child c + 1
+        used += 2;
+        splits++;
+        // Child records: lanes 0..12 / 16..28 fold the left / right bins (binned
+        // split), or the halves' boxes are reduced over their triangles (median).
+        uint32_t word = 0;
+        const uint32_t w = lane & 15, side = lane >> 4;
         uint32_t m = n / 2;
         if (d.kind == 1) {
-            const uint32_t v12 = side_word(sb, d.axis, d.split, lane);
-            if (lane == 12) sh[1] = v12;
+            word = side_word(sb, d.axis, d.split, lane);
+            if (lane == 12) sh[1] = word;
+        } else {
+            uint32_t bx[2][6];
+            for (int s2 = 0; s2 < 2; s2++)
+                for (uint32_t q = 0; q < 6; q++) bx[s2][q] = identity(q);
+            for (uint32_t i = lane; i < n; i += 64) {
+                const uint32_t t = s_ids[rb + i];
+                const float4 l = tlo[t], h = thi[t];
+                const uint32_t e[6] = {enc(l.x), enc(l.y), enc(l.z), enc(h.x), enc(h.y), enc(h.z)};
+                const bool right = i >= m;
+                for (uint32_t q = 0; q < 6; q++) {
+                    const uint32_t a = right ? bx[1][q] : bx[0][q];
+                    const uint32_t r = q < 3 ? min(a, e[q]) : max(a, e[q]);
+                    if (right) bx[1][q] = r;
+                    else bx[0][q] = r;
+                }
+            }
+            for (int s2 = 0; s2 < 2; s2++)
+                for (uint32_t q = 0; q < 6; q++) {
+                    uint32_t x = bx[s2][q];
+                    for (int o = 32; o >= 1; o >>= 1) {
+                        uint32_t y = __shfl_xor(x, o);
+                        x = q < 3 ? min(x, y) : max(x, y);
+                    }
+                    if (side == (uint32_t)s2 && w == q) word = x;
+                }
+            if (w >= 6 && w < 12 && side <= 1) word = pick6(cb, w - 6);
         }
-        if (lane == 0) sh[0] = atomicAdd(&counters[C_NODES], 2u);
         __syncthreads();
-        const uint32_t c = sh[0];
+        if (d.kind == 1) m = sh[1];
+        // Push right (below) and left (on top): records in LDS, and the
+        // temp nodes in global memory for the numbering.
+        uint32_t *R = &stk[sp * kRec], *L = &stk[(sp + 1) * kRec];
+        uint32_t *gL = reinterpret_cast<uint32_t *>(&nodes[c]), *gR = reinterpret_cast<uint32_t *>(&nodes[c + 1]);
+        if (w < 12 && side <= 1) {
+            (side ? R : L)[w] = word;
+            (side ? gR : gL)[w] = word;
+        }
+        if (lane >= 32 && lane < 40) {
+            const uint32_t q = lane - 32, s2 = q >> 2, k = 12 + (q & 3);
+            const uint32_t v = k == 12 ? (s2 ? nb + m : nb)
+                             : k == 13 ? (s2 ? n - m : m)
+                             : k == 14 ? (node << 1 | s2)
+                                       : depth + 1;
+            (s2 ? R : L)[k] = v;
+            (s2 ? gR : gL)[k] = v;
+        }
+        if (lane == 40) nodes[node].depth = depth | kInner;
+        if (lane == 41) {
+            s_node[sp] = c + 1;
+            s_node[sp + 1] = c;
+        }
         if (d.kind == 1) {
-            m = sh[1];
-            // Ballot stable partition of [rb, rb + n) through s_tmp.
-            const float lo = f.clo[d.axis], sc = scale[d.axis];
+            // Ballot stable partition of [rb, rb + n) through the bin area.
+            const float lo = pick3(f.clo, d.axis), sc = pick3(scale, d.axis);
             uint32_t lcount = 0;
-            for (uint32_t base = 0; base < n; base += 64) {
-                const uint32_t i = base + lane;
+            for (uint32_t b2 = 0; b2 < n; b2 += 64) {
+                const uint32_t i = b2 + lane;
                 const bool in = i < n;
                 const uint32_t t = in ? s_ids[rb + i] : 0u;
                 const bool fl = in && bin_of(coord(tlo, thi, tcz, t, d.axis), lo, sc) < (int)d.split;
@@ -680,54 +758,29 @@ __global__ __launch_bounds__(64) void k_small(const uint32_t *small, TNode *node
                 const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
                 if (in) {
                     const uint32_t lb = lcount + rank, dst = fl ? lb : m + (i - lb);
-                    if (dst < n) s_tmp[rb + dst] = t;
+                    if (dst < n) sb[dst] = t;
                 }
                 lcount += (uint32_t)__popcll(mask);
             }
             __syncthreads();
-            for (uint32_t i = lane; i < n; i += 64) s_ids[rb + i] = s_tmp[rb + i];
-        }
-        if (d.kind == 2) {
-            // Boxes of the two halves (positions [0, m) and [m, n)).
-            uint32_t bx[2][6];
-            for (int s = 0; s < 2; s++)
-                for (uint32_t w = 0; w < 6; w++) bx[s][w] = identity(w);
-            for (uint32_t i = lane; i < n; i += 64) {
-                const uint32_t t = s_ids[rb + i];
-                const float4 l = tlo[t], h = thi[t];
-                const uint32_t e[6] = {enc(l.x), enc(l.y), enc(l.z), enc(h.x), enc(h.y), enc(h.z)};
-                const bool right = i >= m;
-                for (uint32_t w = 0; w < 6; w++) {
-                    const uint32_t a = right ? bx[1][w] : bx[0][w];
-                    const uint32_t r = w < 3 ? min(a, e[w]) : max(a, e[w]);
-                    if (right) bx[1][w] = r;
-                    else bx[0][w] = r;
-                }
-            }
-            for (int s = 0; s < 2; s++)
-                for (uint32_t w = 0; w < 6; w++) {
-                    uint32_t x = bx[s][w];
-                    for (int q = 32; q >= 1; q >>= 1) {
-                        uint32_t o = __shfl_xor(x, q);
-                        x = w < 3 ? min(x, o) : max(x, o);
-                    }
-                    if (lane == 0) s_mbox[6 * s + w] = x;
-                }
-            __syncthreads();
-        }
-        write_children(nodes, v, pn, c, d, sb, s_mbox, m, lane);
-        __threadfence_block();
-        if (lane == 0) {
-            stk[sp] = c + 1;  // right below left: left is taken next
-            stk[sp + 1] = c;
+            for (uint32_t i = lane; i < n; i += 64) s_ids[rb + i] = sb[i];
         }
         sp += 2;
         __syncthreads();
     }
     __syncthreads();
     for (uint32_t i = lane; i < n0; i += 64) ids[b0 + i] = s_ids[i];
+    // Reserved slots left unused: empty records the numbering skips.
+    for (uint32_t i = used + lane; i < reserved; i += 64) {
+        nodes[base + i].n = 0;
+        nodes[base + i].depth = 0;
+        nodes[base + i].parent = kNone;
+    }
     for (int q = 32; q >= 1; q >>= 1) max_depth = max(max_depth, __shfl_xor(max_depth, q));
-    if (lane == 0) atomicMax(&counters[C_DEPTH], max_depth);
+    if (lane == 0) {
+        atomicMax(&counters[C_DEPTH], max_depth);
+        if (splits) atomicAdd(&counters[C_INNER], splits);
+    }
 }
 
 // ---- scans ----------------------------------------------------------------
@@ -782,7 +835,7 @@ __global__ void k_rank_count(const TNode *nodes, uint32_t T, uint32_t *cnt, uint
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= T) return;
     const TNode &nd = nodes[v];
-    if (!(nd.depth & kInner)) return;
+    if (nd.n == 0 || !(nd.depth & kInner)) return;
     atomicAdd(&cnt[nd.b], 1u);
     atomicMin(&d0[nd.b], nd.depth & ~kInner);
 }
@@ -791,7 +844,7 @@ __global__ void k_rank(const TNode *nodes, uint32_t T, const uint32_t *base, con
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= T) return;
     const TNode &nd = nodes[v];
-    if (!(nd.depth & kInner)) return;
+    if (nd.n == 0 || !(nd.depth & kInner)) return;
     rank[v] = base[nd.b] + (nd.depth & ~kInner) - d0[nd.b];
 }
 
@@ -808,6 +861,7 @@ __global__ void k_emit_nodes(const TNode *nodes, uint32_t T, uint32_t node_count
     }
     if (v >= T) return;
     const TNode &nd = nodes[v];
+    if (nd.n == 0) return;  // a reserved slot a subtree did not use
     const uint32_t idx = nd.parent == kNone ? 0u : 2u + 2u * rank[nd.parent >> 1] + (nd.parent & 1u);
     if (idx >= node_count) return;  // ranks are < inner count by construction
     const bool inner = (nd.depth & kInner) != 0;
@@ -954,18 +1008,29 @@ __device__ inline float wide_area(const uint32_t *words, uint32_t node) {
 }
 
 // One thread per binary node that roots a wide node of this level.
-__global__ void k_wide_level(const uint32_t *words, const uint32_t *bstart, const uint32_t *cur, uint32_t n_cur,
-                             uint32_t depth, uint32_t root_leaf, uint32_t *next, uint32_t *wkids, uint32_t *wdepth,
-                             uint32_t *wb, uint32_t *wide_of_bin, uint32_t *counters) {
+// max_w: capacity of the wide arrays and level lists (the binary inner
+// node count); a tree that would exceed it is flagged, never written past.
+__global__ void k_wide_level(const uint32_t *words, uint32_t node_count, const uint32_t *bstart, const uint32_t *cur,
+                             uint32_t n_cur, uint32_t depth, uint32_t root_leaf, uint32_t max_w, uint32_t *next,
+                             uint32_t *wkids, uint32_t *wdepth, uint32_t *wb, uint32_t *wide_of_bin,
+                             uint32_t *counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_cur) return;
     const uint32_t s = cur[i];
+    if (s >= node_count) {
+        counters[C_WIDE_BAD] = 2;
+        return;
+    }
     uint32_t kids[4] = {kNone, kNone, kNone, kNone}, nk;
     if (root_leaf) {
         kids[0] = s;
         nk = 1;
     } else {
         const uint32_t f = words[8 * (size_t)s + 3];
+        if (f < 2 || f + 1 >= node_count) {  // an inner node's children are a pair after the root
+            counters[C_WIDE_BAD] = 2;
+            return;
+        }
         kids[0] = f;
         kids[1] = f + 1;
         nk = 2;
@@ -983,20 +1048,33 @@ __global__ void k_wide_level(const uint32_t *words, const uint32_t *bstart, cons
         }
         if (best < 0) break;
         const uint32_t f = words[8 * (size_t)kids[best] + 3];
+        if (f < 2 || f + 1 >= node_count) {
+            counters[C_WIDE_BAD] = 2;
+            return;
+        }
         for (int j = (int)nk; j > best + 1; j--) kids[j] = kids[j - 1];
         kids[best] = f;
         kids[best + 1] = f + 1;
         nk++;
     }
     const uint32_t w = atomicAdd(&counters[C_WIDE], 1u);
+    if (w >= max_w) {
+        counters[C_WIDE_BAD] = 2;
+        return;
+    }
     wdepth[w] = depth;
     wb[w] = bstart[s];
     wide_of_bin[s] = w;
     for (uint32_t j = 0; j < 4; j++) wkids[4 * (size_t)w + j] = kids[j];
     for (uint32_t j = 0; j < nk; j++) {
         const uint32_t c = words[8 * (size_t)kids[j] + 7];
-        if (c == 0) next[atomicAdd(&counters[C_WIDE_NEXT], 1u)] = kids[j];
-        else if (c > kWideLeafMax) counters[C_WIDE_BAD] = 1;
+        if (c == 0) {
+            const uint32_t at = atomicAdd(&counters[C_WIDE_NEXT], 1u);
+            if (at < max_w) next[at] = kids[j];
+            else counters[C_WIDE_BAD] = 2;
+        } else if (c > kWideLeafMax) {
+            counters[C_WIDE_BAD] = 1;
+        }
     }
     atomicMax(&counters[C_WIDE_DEPTH], depth);
 }
@@ -1186,8 +1264,8 @@ int build_geometry(const SoupDev &soup, uint32_t n_emissive, uint32_t *emissive_
     if (hc[C_SMALL] > 0) k_small<<<hc[C_SMALL], 64, 0, s>>>(small, nodes, ids, tlo, thi, tcz, counters);
     if ((rc = launch_check(err, "small subtrees"))) return rc;
     if ((rc = read_counters(counters, hc, s, err))) return rc;
-    const uint32_t T = hc[C_NODES];
-    const uint32_t inner = (T - 1) / 2;
+    const uint32_t T = hc[C_NODES];  // temp slots, unused reserved ones included
+    const uint32_t inner = hc[C_INNER];
     const uint32_t node_count = 2 + 2 * inner;
     out->bvh_depth = hc[C_DEPTH];
 
@@ -1217,13 +1295,22 @@ int build_geometry(const SoupDev &soup, uint32_t n_emissive, uint32_t *emissive_
         return rc;
     k_fill<<<1, 64, 0, s>>>(wcur, 1, 0u);  // the binary root roots the wide tree
     uint32_t n_cur = 1, depth = 1;
-    const uint32_t root_leaf = T == 1 ? 1u : 0u;
+    const uint32_t root_leaf = inner == 0 ? 1u : 0u;  // T also counts reserved slots left unused
     while (n_cur > 0) {
+        if (depth > kMaxDepth + 1) {
+            err = "device BVH build: 4-wide collapse deeper than the binary tree";
+            return OXPT_E_BACKEND;
+        }
         k_fill<<<1, 64, 0, s>>>(counters + C_WIDE_NEXT, 1, 0u);
-        k_wide_level<<<blocks(n_cur, 256), 256, 0, s>>>(words, bstart, wcur, n_cur, depth, depth == 1 ? root_leaf : 0u,
-                                                        wnext, wkids, wdepth, wb, wide_of_bin, counters);
+        k_wide_level<<<blocks(n_cur, 256), 256, 0, s>>>(words, node_count, bstart, wcur, n_cur, depth,
+                                                        depth == 1 ? root_leaf : 0u,
+                                                        max_wide, wnext, wkids, wdepth, wb, wide_of_bin, counters);
         if ((rc = launch_check(err, "wide levels"))) return rc;
         if ((rc = read_counters(counters, hc, s, err))) return rc;
+        if (hc[C_WIDE_BAD] == 2 || hc[C_WIDE] > max_wide || hc[C_WIDE_NEXT] > max_wide) {
+            err = "device BVH build: 4-wide collapse exceeds the binary tree's inner nodes";
+            return OXPT_E_BACKEND;
+        }
         std::swap(wcur, wnext);
         n_cur = hc[C_WIDE_NEXT];
         depth++;
