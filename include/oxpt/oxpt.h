@@ -271,7 +271,12 @@ typedef struct oxpt_tuning {
                                 one path per claim (then added in sample order; at most 2^28 buffered
                                 samples per launch): 0 = default (8, half the frame), 1..16, or
                                 OXPT_TAIL_NONE (every pixel as one run of samples) */
+    uint32_t early;          /* scenes without punctual lights: 0 = default, the megakernel forms built to
+                                make the NEE shadow ray and the BSDF continuation together at a hit (the
+                                BSDF does not live across traversals); OXPT_EARLY_OFF = the general
+                                forms. Same images either way */
 } oxpt_tuning;
+#define OXPT_EARLY_OFF 0xFFFFu
 #define OXPT_STACK_PRIVATE 0xFFFFu
 #define OXPT_TAIL_NONE 0xFFFFu
 int oxpt_ctx_set_tuning(oxpt_ctx *ctx, const oxpt_tuning *t);

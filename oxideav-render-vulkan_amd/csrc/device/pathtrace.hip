@@ -104,6 +104,10 @@ struct PathState {
     bool front;
     float4 pl_u;
     uint32_t job;
+    // Early form (no punctual lights): the continuation ray's origin, sampled
+    // at the hit, and whether a continuation follows the NEE shadow ray.
+    F3 c_o;
+    bool cont;
 };
 
 // Camera ray of sample `index` of pixel (x, y) (wgsl:1715-1764).
@@ -152,20 +156,202 @@ __device__ __forceinline__ void init_path(const oxpt_params &P, const uint32_t *
     st.front = true;
     st.pl_u = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     st.job = 0;
+    st.c_o = f3(0.0f);
+    st.cont = false;
+}
+
+// The shadow ray of NEE job `job` at a surface vertex: punctual light
+// job < n_punct, then the emissive / environment set (wgsl:1863-1997).
+// ok with its ray and the contribution it adds when unoccluded.
+struct NeeRay {
+    bool ok;
+    F3 o, d, pending;
+    float tmax;
+    uint32_t id, skip;
+};
+
+__device__ __forceinline__ NeeRay nee_job(const oxpt_params &P, const Scene &S, const Bsdf &bsdf, F3 p, F3 ng,
+                                          float4 pl_u, F3 beta, uint32_t k, uint32_t job) {
+    const uint32_t strategy = P.cfg[1];
+    const bool has_env = P.env[2] != 0;
+    const uint32_t n_elights = P.cfg[3], n_punct = P.cfg[2];
+    const float clampv = P.fparams[1];
+    const uint32_t ray_id = k << 16u;
+    NeeRay out;
+    out.ok = false;
+    bool ok = false;
+    F3 l = f3(0.0f);
+    float tmax = INF;
+    uint32_t rid = ray_id | 0xffffu;
+    uint32_t skip = NONE;
+    F3 le = f3(0.0f);
+    float pl = -1.0f;
+    if (job < n_punct) {
+        LightSample ls = punctual_sample(S, job, p);
+        if (ls.ok) {
+            ok = true;
+            l = ls.l;
+            if (ls.distance >= 0.0f) tmax = ls.distance * (1.0f - 1.0e-4f);
+            rid = ray_id | 0x8000u | (job & 0x7fffu);
+            le = ls.radiance;
+        }
+    } else if (strategy != 2 && (has_env || n_elights > 0)) {
+        float pe = S.p_env;
+        if (pl_u.z < pe) {
+            if (has_env) {
+                EnvSample es = env_sample(S, pl_u.x, pl_u.y);
+                if (es.ok && pe * es.pdf > 0.0f) {
+                    ok = true;
+                    l = es.d;
+                    le = es.le;
+                    pl = pe * es.pdf;
+                }
+            }
+        } else if (n_elights > 0) {
+            float us = pl_u.z;
+            if (pe > 0.0f) us = clampf((pl_u.z - pe) / (1.0f - pe), 0.0f, 0.999999f);
+            uint32_t li = pick_elight(S, us);
+            uint32_t lslot = elight_slot(S, li);
+            F3 a = tri_pos(S, lslot, 0), b = tri_pos(S, lslot, 1), c = tri_pos(S, lslot, 2);
+            bool has_target = false;
+            F3 q = f3(0.0f);
+            const float omega = triangle_solid_angle(p, a, b, c);
+            if (omega >= MIN_SPHERICAL_SOLID_ANGLE) {
+                F3 sl = sample_spherical_triangle(p, a, b, c, pl_u.x, pl_u.y);
+                if (dot(sl, sl) > 0.0f) {
+                    F3 n = cross(b - a, c - a);
+                    float dn = dot(sl, n);
+                    if (dn != 0.0f) {
+                        float t = dot(a - p, n) / dn;
+                        if (t > 0.0f && finite(t)) {
+                            q = p + sl * t;
+                            has_target = true;
+                        }
+                    }
+                }
+            } else {
+                float su = sqrtf(pl_u.x);
+                F3 w3 = {1.0f - su, su * (1.0f - pl_u.y), su * pl_u.y};
+                q = a * w3.x + b * w3.y + c * w3.z;
+                has_target = true;
+            }
+            if (has_target) {
+                F3 e1 = b - a, e2 = c - a, rr = q - a;
+                float d11 = dot(e1, e1), d12 = dot(e1, e2), d22 = dot(e2, e2);
+                float r1 = dot(rr, e1), r2 = dot(rr, e2);
+                float det = d11 * d22 - d12 * d12;
+                F3 to = q - p;
+                float dist2 = dot(to, to);
+                if (finite(det) && fabsf(det) > 1.17549435e-38f && dist2 > 1.0e-12f) {
+                    float bu = (d22 * r1 - d12 * r2) / det;
+                    float bv = (d11 * r2 - d12 * r1) / det;
+                    F3 lb = vmin(vmax(F3{1.0f - bu - bv, bu, bv}, f3(0.0f)), f3(1.0f));
+                    F3 lng = surface(S, lslot, lb).ng;
+                    bool lds = (mat_flags(S, tri_material(S, lslot)) & 4u) != 0;
+                    float dist = sqrtf(dist2);
+                    F3 ld = to / dist;
+                    float cos_l = -dot(lng, ld);
+                    float tp = tri_pdf_omega(S, li, a, b, c, omega, p, q);
+                    if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && tp > 0.0f && finite(tp)) {
+                        ok = true;
+                        l = ld;
+                        le = emission(S, lslot, lb);
+                        pl = tp;
+                        tmax = fmaxf(dist * (1.0f - 1.0e-4f), 0.0f);
+                        skip = tri_global(S, lslot);
+                    }
+                }
+            }
+        }
+    }
+    if (ok) {
+        Eval e = bsdf_eval(S.lds_sheen, bsdf, l);
+        if (!is_black(e.f) && !is_black(le)) {
+            if (pl < 0.0f) {
+                out.pending = clamp_c(clampv, beta * e.f * le);
+            } else {
+                float w = 1.0f;
+                if (strategy == 0) w = power_heuristic(pl, e.pdf);
+                out.pending = clamp_c(clampv, beta * e.f * le * (w / pl));
+            }
+            out.ok = true;
+            out.o = spawn(p, ng, l);
+            out.d = l;
+            out.tmax = tmax;
+            out.id = rid;
+            out.skip = skip;
+        }
+    }
+    return out;
+}
+
+// BSDF sampling at vertex k (wgsl:1999-2039): beta, the previous pdf and
+// position, the medium on entering a thick transmissive surface, roulette
+// from rr_start, then k + 1 and the new direction. False when the path ends;
+// otherwise *origin is the continuation ray's origin.
+__device__ __forceinline__ bool bsdf_continue(const oxpt_params &P, const uint32_t *lds_sobol, const float *lds_sheen,
+                                              PathState &st, const Bsdf &bsdf, F3 p, F3 ng, float thickness,
+                                              float transmission, F3 atten_color, float atten_dist, bool front,
+                                              F3 *origin) {
+    float4 pb_u = pattern(lds_sobol, st.pixel_seed, st.index, 1u + 2u * st.k);
+    BSample bs = bsdf_sample(lds_sheen, bsdf, F3{pb_u.x, pb_u.y, pb_u.z});
+    if (!bs.ok) return false;
+    st.beta = st.beta * (bs.f * (1.0f / bs.pdf));
+    if (!finite3(st.beta)) return false;
+    st.prev_pdf = bs.pdf;
+    st.prev_pos = p;
+    if (dot(bs.l, ng) < 0.0f && thickness > 0.0f && transmission > 0.0f) {
+        if (front && atten_dist >= 0.0f) {
+            F3 lc = vmax(atten_color, f3(1.0e-6f));
+            st.sigma = F3{-logf(lc.x), -logf(lc.y), -logf(lc.z)} / atten_dist;
+            st.in_medium = true;
+        } else {
+            st.in_medium = false;
+        }
+    }
+    if (st.k + 1u >= P.cfg[0]) {
+        float q = clampf(max3(st.beta), 0.05f, 1.0f);
+        if (q < 1.0f) {
+            if (pb_u.w >= q) return false;
+            st.beta = st.beta * (1.0f / q);
+        }
+    }
+    st.k += 1;
+    st.d = bs.l;
+    *origin = spawn(p, ng, bs.l);
+    return true;
 }
 
 // Advances the state machine past the ray just traced, given its hit
 // (wgsl:1766-2040). Returns true when the path ended; *covered says
 // whether the camera ray hit anything. Otherwise st.r_* holds the next ray.
+//
+// Early = true is the form for scenes without punctual lights (one NEE job
+// per vertex): the NEE shadow ray and the BSDF continuation are both made at
+// the hit, so the BSDF and the vertex need not live across a traversal; the
+// shadow ray is traced first, then the continuation. Every random number and
+// every sum is the same, so the images are bit-identical to the general form.
+template <bool Early>
 __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol, PathState &st,
                         const Hit &hit, bool *covered) {
     const uint32_t strategy = P.cfg[1];
     const bool has_env = P.env[2] != 0;
-    const uint32_t n_elights = P.cfg[3], n_punct = P.cfg[2];
+    const uint32_t n_punct = P.cfg[2];
     const float clampv = P.fparams[1];
     *covered = true;
     if (st.shadow) {
         if (hit.slot == NONE) st.radiance = st.radiance + st.pending;
+        if constexpr (Early) {
+            // The continuation was sampled at the hit; trace it now.
+            if (!st.cont) return true;
+            st.shadow = false;
+            st.r_o = st.c_o;
+            st.r_d = st.d;
+            st.r_tmax = INF;
+            st.r_id = st.k << 16u;
+            st.r_skip = NONE;
+            return false;
+        }
         st.job += 1;
     } else {
         if (st.in_medium) {
@@ -225,6 +411,36 @@ __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *ld
             return true;
         }
         if (st.k >= P.dims[3]) return true;
+        if constexpr (Early) {
+            const Bsdf bsdf = make_bsdf(S.lds_sheen, mat, -st.d, surf.ng, hit.front);
+            if (bsdf_black(bsdf)) return true;
+            const F3 p = surf.position, ng = bsdf.ng;
+            const NeeRay nr = nee_job(P, S, bsdf, p, ng, pattern(lds_sobol, st.pixel_seed, st.index, 2u + 2u * st.k),
+                                      st.beta, st.k, n_punct);
+            F3 co = f3(0.0f);
+            const bool cont = bsdf_continue(P, lds_sobol, S.lds_sheen, st, bsdf, p, ng, mat.thickness, mat.transmission,
+                                            mat.atten_color, mat.atten_dist, hit.front, &co);
+            if (nr.ok) {
+                st.pending = nr.pending;
+                st.r_o = nr.o;
+                st.r_d = nr.d;
+                st.r_tmax = nr.tmax;
+                st.r_id = nr.id;
+                st.r_skip = nr.skip;
+                st.shadow = true;
+                st.cont = cont;
+                st.c_o = co;
+                return false;
+            }
+            if (!cont) return true;
+            st.shadow = false;
+            st.r_o = co;
+            st.r_d = st.d;
+            st.r_tmax = INF;
+            st.r_id = st.k << 16u;
+            st.r_skip = NONE;
+            return false;
+        }
         st.bsdf = make_bsdf(S.lds_sheen, mat, -st.d, surf.ng, hit.front);
         if (bsdf_black(st.bsdf)) return true;
         st.p = surf.position;
@@ -239,143 +455,28 @@ __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *ld
     }
 
     // ---- Next NEE shadow ray: punctual j < n, then the area set (wgsl:1863-1997).
-    const uint32_t ray_id = st.k << 16u;
     while (st.job <= n_punct) {
-        bool ok = false;
-        F3 l = f3(0.0f);
-        float tmax = INF;
-        uint32_t rid = ray_id | 0xffffu;
-        uint32_t skip = NONE;
-        F3 le = f3(0.0f);
-        float pl = -1.0f;
-        if (st.job < n_punct) {
-            LightSample ls = punctual_sample(S, st.job, st.p);
-            if (ls.ok) {
-                ok = true;
-                l = ls.l;
-                if (ls.distance >= 0.0f) tmax = ls.distance * (1.0f - 1.0e-4f);
-                rid = ray_id | 0x8000u | (st.job & 0x7fffu);
-                le = ls.radiance;
-            }
-        } else if (strategy != 2 && (has_env || n_elights > 0)) {
-            float pe = S.p_env;
-            if (st.pl_u.z < pe) {
-                if (has_env) {
-                    EnvSample es = env_sample(S, st.pl_u.x, st.pl_u.y);
-                    if (es.ok && pe * es.pdf > 0.0f) {
-                        ok = true;
-                        l = es.d;
-                        le = es.le;
-                        pl = pe * es.pdf;
-                    }
-                }
-            } else if (n_elights > 0) {
-                float us = st.pl_u.z;
-                if (pe > 0.0f) us = clampf((st.pl_u.z - pe) / (1.0f - pe), 0.0f, 0.999999f);
-                uint32_t li = pick_elight(S, us);
-                uint32_t lslot = elight_slot(S, li);
-                F3 a = tri_pos(S, lslot, 0), b = tri_pos(S, lslot, 1), c = tri_pos(S, lslot, 2);
-                bool has_target = false;
-                F3 q = f3(0.0f);
-                const float omega = triangle_solid_angle(st.p, a, b, c);
-                if (omega >= MIN_SPHERICAL_SOLID_ANGLE) {
-                    F3 sl = sample_spherical_triangle(st.p, a, b, c, st.pl_u.x, st.pl_u.y);
-                    if (dot(sl, sl) > 0.0f) {
-                        F3 n = cross(b - a, c - a);
-                        float dn = dot(sl, n);
-                        if (dn != 0.0f) {
-                            float t = dot(a - st.p, n) / dn;
-                            if (t > 0.0f && finite(t)) {
-                                q = st.p + sl * t;
-                                has_target = true;
-                            }
-                        }
-                    }
-                } else {
-                    float su = sqrtf(st.pl_u.x);
-                    F3 w3 = {1.0f - su, su * (1.0f - st.pl_u.y), su * st.pl_u.y};
-                    q = a * w3.x + b * w3.y + c * w3.z;
-                    has_target = true;
-                }
-                if (has_target) {
-                    F3 e1 = b - a, e2 = c - a, rr = q - a;
-                    float d11 = dot(e1, e1), d12 = dot(e1, e2), d22 = dot(e2, e2);
-                    float r1 = dot(rr, e1), r2 = dot(rr, e2);
-                    float det = d11 * d22 - d12 * d12;
-                    F3 to = q - st.p;
-                    float dist2 = dot(to, to);
-                    if (finite(det) && fabsf(det) > 1.17549435e-38f && dist2 > 1.0e-12f) {
-                        float bu = (d22 * r1 - d12 * r2) / det;
-                        float bv = (d11 * r2 - d12 * r1) / det;
-                        F3 lb = vmin(vmax(F3{1.0f - bu - bv, bu, bv}, f3(0.0f)), f3(1.0f));
-                        F3 lng = surface(S, lslot, lb).ng;
-                        bool lds = (mat_flags(S, tri_material(S, lslot)) & 4u) != 0;
-                        float dist = sqrtf(dist2);
-                        F3 ld = to / dist;
-                        float cos_l = -dot(lng, ld);
-                        float tp = tri_pdf_omega(S, li, a, b, c, omega, st.p, q);
-                        if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && tp > 0.0f && finite(tp)) {
-                            ok = true;
-                            l = ld;
-                            le = emission(S, lslot, lb);
-                            pl = tp;
-                            tmax = fmaxf(dist * (1.0f - 1.0e-4f), 0.0f);
-                            skip = tri_global(S, lslot);
-                        }
-                    }
-                }
-            }
-        }
-        if (ok) {
-            Eval e = bsdf_eval(S.lds_sheen, st.bsdf, l);
-            if (!is_black(e.f) && !is_black(le)) {
-                if (pl < 0.0f) {
-                    st.pending = clamp_c(clampv, st.beta * e.f * le);
-                } else {
-                    float w = 1.0f;
-                    if (strategy == 0) w = power_heuristic(pl, e.pdf);
-                    st.pending = clamp_c(clampv, st.beta * e.f * le * (w / pl));
-                }
-                st.r_o = spawn(st.p, st.ng, l);
-                st.r_d = l;
-                st.r_tmax = tmax;
-                st.r_id = rid;
-                st.r_skip = skip;
-                st.shadow = true;
-                return false;
-            }
+        const NeeRay nr = nee_job(P, S, st.bsdf, st.p, st.ng, st.pl_u, st.beta, st.k, st.job);
+        if (nr.ok) {
+            st.pending = nr.pending;
+            st.r_o = nr.o;
+            st.r_d = nr.d;
+            st.r_tmax = nr.tmax;
+            st.r_id = nr.id;
+            st.r_skip = nr.skip;
+            st.shadow = true;
+            return false;
         }
         st.job += 1;
     }
 
     // ---- BSDF sampling -> next path ray (wgsl:1999-2039).
-    float4 pb_u = pattern(lds_sobol, st.pixel_seed, st.index, 1u + 2u * st.k);
-    BSample bs = bsdf_sample(S.lds_sheen, st.bsdf, F3{pb_u.x, pb_u.y, pb_u.z});
-    if (!bs.ok) return true;
-    st.beta = st.beta * (bs.f * (1.0f / bs.pdf));
-    if (!finite3(st.beta)) return true;
-    st.prev_pdf = bs.pdf;
-    st.prev_pos = st.p;
-    if (dot(bs.l, st.ng) < 0.0f && st.thickness > 0.0f && st.transmission > 0.0f) {
-        if (st.front && st.atten_dist >= 0.0f) {
-            F3 lc = vmax(st.atten_color, f3(1.0e-6f));
-            st.sigma = F3{-logf(lc.x), -logf(lc.y), -logf(lc.z)} / st.atten_dist;
-            st.in_medium = true;
-        } else {
-            st.in_medium = false;
-        }
-    }
-    if (st.k + 1u >= P.cfg[0]) {
-        float q = clampf(max3(st.beta), 0.05f, 1.0f);
-        if (q < 1.0f) {
-            if (pb_u.w >= q) return true;
-            st.beta = st.beta * (1.0f / q);
-        }
-    }
-    st.k += 1;
-    st.d = bs.l;
+    F3 co = f3(0.0f);
+    if (!bsdf_continue(P, lds_sobol, S.lds_sheen, st, st.bsdf, st.p, st.ng, st.thickness, st.transmission,
+                       st.atten_color, st.atten_dist, st.front, &co))
+        return true;
     st.shadow = false;
-    st.r_o = spawn(st.p, st.ng, bs.l);
+    st.r_o = co;
     st.r_d = st.d;
     st.r_tmax = INF;
     st.r_id = st.k << 16u;
@@ -386,12 +487,13 @@ __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *ld
 // One iteration: trace the current ray and advance the state machine.
 // `lds_stack` / `lds_depth`: this wave's LDS share of the traversal stack
 // (depth 0: the whole stack in private memory).
+template <bool Early = false>
 __device__ __forceinline__ bool step_path(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol,
                                           PathState &st, bool *covered, lds_u64 *lds_stack = nullptr,
                                           uint32_t lds_depth = 0) {
     Hit hit = traverse<-1>(S, make_ray(st.r_o, st.r_d), st.r_tmax, st.path_seed, st.r_id, !st.shadow && st.k == 0,
                            st.shadow, st.r_skip, lds_stack, lds_depth);
-    return advance(P, S, lds_sobol, st, hit, covered);
+    return advance<Early>(P, S, lds_sobol, st, hit, covered);
 }
 
 __device__ __forceinline__ void stage_tables(const uint32_t *tables, const oxpt_params &P, uint32_t *lds_sobol,
@@ -413,7 +515,7 @@ __device__ __forceinline__ void item_pixel(uint32_t tiles_x, uint32_t item, uint
 // VGPRs, more latency hiding, possible spills): an autotuner knob.
 // lds_depth: traversal-stack entries per lane kept in dynamic LDS
 // (blockDim / 64 * lds_depth * 512 bytes), the rest in private memory.
-template <int MinWaves>
+template <int MinWaves, bool Early = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves > 0 ? MinWaves : 1)))
 trace_persistent(TraceArgs A, uint32_t lds_depth) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
@@ -497,7 +599,7 @@ trace_persistent(TraceArgs A, uint32_t lds_depth) {
         // before returning to the claim step, so the lanes of a wave stay in
         // the same phase of their bounces (less divergent shading).
         bool covered, ended;
-        while (!(ended = step_path(A.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) && st.shadow &&
+        while (!(ended = step_path<Early>(A.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) && st.shadow &&
                A.P.cfg[2] != 0) {
         }
         if (ended) {
@@ -575,7 +677,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) tr
 // launch's sample buffer; batch_reduce then adds every pixel's samples into
 // its accumulator in sample order, so every job's image equals its image
 // alone. Jobs are ordered largest first by the host.
-template <int MinWaves>
+template <int MinWaves, bool Early = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves)))
 trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
@@ -634,7 +736,7 @@ trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
             }
             if (!has_path) continue;
             bool covered, ended;
-            while (!(ended = step_path(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) && st.shadow &&
+            while (!(ended = step_path<Early>(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) && st.shadow &&
                    J.P.cfg[2] != 0) {
             }
             if (ended) {
@@ -801,8 +903,10 @@ namespace oxd {
 // Kernel forms (oxpt_tuning.kernel): 0 the default (4 waves per SIMD),
 // 1 plain per-pixel, 2..6 persistent built for 3, 4, 5, 6, 8 waves per SIMD,
 // 8 persistent built for 2 waves per SIMD (256 VGPRs).
+// early: the scene has no punctual lights, so the forms built with the
+// early continuation (3, 4 and 5 waves per SIMD) apply; same images.
 hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, uint32_t lds_depth,
-                        hipStream_t s) {
+                        bool early, hipStream_t s) {
     if (kernel == 1) {
         uint32_t blocks = args.n_items / 64u;
         hipLaunchKernelGGL(trace_tiles, dim3(blocks), dim3(64), 0, s, args);
@@ -812,7 +916,11 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
-    switch (kernel) {
+    if (early && (kernel == 0 || kernel == 2 || kernel == 3 || kernel == 4)) {
+        if (kernel == 2) hipLaunchKernelGGL((trace_persistent<3, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
+        else if (kernel == 4) hipLaunchKernelGGL((trace_persistent<5, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
+        else hipLaunchKernelGGL((trace_persistent<4, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
+    } else switch (kernel) {
         case 2: hipLaunchKernelGGL(trace_persistent<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 4: hipLaunchKernelGGL(trace_persistent<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 5: hipLaunchKernelGGL(trace_persistent<6>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
@@ -830,12 +938,16 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
 
 // Batched forms: the same numbering (default: the 4-waves build).
 hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid,
-                              uint32_t lds_depth, hipStream_t s) {
+                              uint32_t lds_depth, bool early, hipStream_t s) {
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, s, args.counters, args.n_jobs, args.max_jobs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
-    switch (kernel) {
+    if (early && (kernel == 0 || kernel == 2 || kernel == 3 || kernel == 4)) {
+        if (kernel == 2) hipLaunchKernelGGL((trace_batch<3, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
+        else if (kernel == 4) hipLaunchKernelGGL((trace_batch<5, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
+        else hipLaunchKernelGGL((trace_batch<4, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
+    } else switch (kernel) {
         case 2: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
         case 4: hipLaunchKernelGGL(trace_batch<5>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
         case 5: hipLaunchKernelGGL(trace_batch<6>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;

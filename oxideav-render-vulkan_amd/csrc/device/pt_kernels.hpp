@@ -58,10 +58,12 @@ struct BatchArgs {
 
 // lds_depth: traversal-stack entries per lane held in LDS (dynamic shared
 // memory, block / 64 * lds_depth * 512 bytes per workgroup).
+// early: every job (scene) has no punctual lights; the forms built for that
+// case make the BSDF continuation at the hit (bit-identical images).
 hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid,
-                              uint32_t lds_depth, hipStream_t s);
+                              uint32_t lds_depth, bool early, hipStream_t s);
 hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, uint32_t lds_depth,
-                        hipStream_t s);
+                        bool early, hipStream_t s);
 hipError_t launch_resolve(const float4 *accum, uint32_t n, const oxpt_resolve_params &rp, void *out, hipStream_t s);
 hipError_t launch_debug_pattern(const uint32_t *tables, uint32_t off_sobol, uint32_t seed, uint32_t n,
                                 const uint32_t *x, const uint32_t *y, const uint32_t *index, const uint32_t *p,

@@ -73,7 +73,7 @@ struct oxpt_ctx {
     std::string summary;
     uint64_t budget = 0, in_use = 0;
     uint32_t n_cu = 0;
-    oxpt_tuning tuning{256, 32, 1u << 30, 0, 0, 0, 0, 0};
+    oxpt_tuning tuning{256, 32, 1u << 30, 0, 0, 0, 0, 0, 0};
     uint32_t *d_work = nullptr;
     oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[job_cap]
     uint32_t *d_counters = nullptr;  // job_cap counters + 1 count word
@@ -81,6 +81,7 @@ struct oxpt_ctx {
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;  // captured batched step (reset + trace)
     uint32_t graph_block = 0, graph_grid = 0, graph_kernel = 0, graph_lds = 0;
+    bool graph_early = false;
     const char *last_kernel = "";  // name of the last trace kernel launched
     float4 *d_samples = nullptr;  // batched launch: one (rgb, covered) per path of a round
     size_t samples_bytes = 0;
@@ -535,11 +536,14 @@ int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
     if (t->bvh != 0 && t->bvh != 2) return fail(c, OXPT_E_ARG, "tuning: bvh must be 0 (4-wide) or 2 (binary)");
     if (t->tail > 16 && t->tail != OXPT_TAIL_NONE)
         return fail(c, OXPT_E_ARG, "tuning: tail must be 0 (default), 1..16 or OXPT_TAIL_NONE");
+    if (t->early != 0 && t->early != OXPT_EARLY_OFF)
+        return fail(c, OXPT_E_ARG, "tuning: early must be 0 (default) or OXPT_EARLY_OFF");
     c->tuning.kernel = t->kernel;
     c->tuning.wf_variant = t->wf_variant;
     c->tuning.stack_lds = t->stack_lds;
     c->tuning.bvh = t->bvh;
     c->tuning.tail = t->tail;
+    c->tuning.early = t->early;
     return OXPT_OK;
 }
 int oxpt_ctx_set_graph(oxpt_ctx *c, int enable) {
@@ -1036,8 +1040,9 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
-        c->last_kernel = c->tuning.kernel == 1 ? "trace_tiles" : "trace_persistent";
-        hipError_t e = oxd::launch_trace(args, c->tuning.kernel, block, grid, lds, c->stream);
+        const bool early = c->tuning.early != OXPT_EARLY_OFF && args.P.cfg[2] == 0 && c->tuning.kernel != 1;
+        c->last_kernel = c->tuning.kernel == 1 ? "trace_tiles" : early ? "trace_persistent(early)" : "trace_persistent";
+        hipError_t e = oxd::launch_trace(args, c->tuning.kernel, block, grid, lds, early, c->stream);
         if (e != hipSuccess) return hip_fail(c, "trace launch", e);
         if (timed) {
             (void)hipEventRecord(ep.b, c->stream);
@@ -1199,13 +1204,15 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
             e = hipMemcpyAsync(c->d_counters + c->job_cap, &nj, 4, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch job upload", e);
         oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap, c->d_samples};
-        c->last_kernel = "trace_batch";
+        bool early = c->tuning.early != OXPT_EARLY_OFF;
+        for (const oxd::Job &J : sorted) early = early && J.P.cfg[2] == 0;
+        c->last_kernel = early ? "trace_batch(early)" : "trace_batch";
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
         if (c->use_graph) {
             if (c->graph && (c->graph_block != block || c->graph_grid != grid || c->graph_kernel != form.kernel ||
-                             c->graph_lds != lds)) {
+                             c->graph_lds != lds || c->graph_early != early)) {
                 (void)hipGraphExecDestroy(c->graph);
                 c->graph = nullptr;
             }
@@ -1215,7 +1222,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
                 if (e == hipSuccess) {
-                    hipError_t le = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, c->stream);
+                    hipError_t le = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, early, c->stream);
                     e = hipStreamEndCapture(c->stream, &g);
                     if (e == hipSuccess) e = le;
                 }
@@ -1226,11 +1233,12 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 c->graph_grid = grid;
                 c->graph_kernel = form.kernel;
                 c->graph_lds = lds;
+                c->graph_early = early;
             }
             e = hipGraphLaunch(c->graph, c->stream);
             c->graph_launches++;
         } else {
-            e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, c->stream);
+            e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, early, c->stream);
         }
         if (e != hipSuccess) return hip_fail(c, "batch launch", e);
         if (timed) {

@@ -144,11 +144,12 @@ class COptions(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block", u32), ("waves_per_cu", u32), ("max_paths", u32), ("kernel", u32), ("wf_variant", u32),
-                ("stack_lds", u32), ("bvh", u32), ("tail", u32)]
+                ("stack_lds", u32), ("bvh", u32), ("tail", u32), ("early", u32)]
 
 
 STACK_PRIVATE = 0xFFFF  # oxpt_tuning.stack_lds: whole traversal stack in private memory
 TAIL_NONE = 0xFFFF  # oxpt_tuning.tail: persistent kernel traces every pixel as a whole run
+EARLY_OFF = 0xFFFF  # oxpt_tuning.early: the general megakernel forms even without punctual lights
 
 
 class BatchItem(C.Structure):
@@ -637,8 +638,8 @@ class GpuPathTracer:
         return lib.oxpt_ctx_last_kernel(self.ctx).decode()
 
     def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0,
-                   tail=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail)
+                   tail=0, early=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail, early)
         rc = lib.oxpt_ctx_set_tuning(self.ctx, C.byref(t))
         if rc:
             _raise(rc, lib.oxpt_last_error(self.ctx).decode())
@@ -776,8 +777,8 @@ class Device:
         return n.value, ms.value
 
     def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0,
-                   tail=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail)
+                   tail=0, early=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail, early)
         self._check(lib.oxpt_ctx_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def get_tuning(self) -> Tuning:

@@ -285,3 +285,40 @@ def test_autotune_keeps_the_image(gpu):
     after = gpu.hdr().view(np.uint32)
     gpu.set_tuning(kernel=0)
     assert np.array_equal(before, after), f"tuning {t.kernel}/{t.stack_lds}"
+
+
+def test_early_continuation_forms_bitwise(gpu):
+    """Without punctual lights the default megakernel forms make the NEE
+    shadow ray and the BSDF continuation together at a hit (the BSDF is not
+    kept across traversals): the same random numbers and sums as the general
+    forms, so identical accumulators — layered materials (volumes,
+    transmission, clearcoat, sheen, blending), environment + emissive NEE,
+    single refine and batched."""
+    scene = oxpt.Scene.layered(seed=4, n_tris=30000, n_textures=2, tex_size=64, n_lights=0)
+    env = oxpt.EnvironmentMap.sky(64, 32, 3)
+    opts = oxpt.RenderOptions(width=80, height=56, samples_per_pixel=5, max_bounces=10)
+    gpu.set_environment(env)
+    gpu.set_tuning(kernel=0, early=oxpt.EARLY_OFF)
+    ref = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+    assert "(early)" not in gpu.last_kernel()
+    for kernel in (0, 2, 4):
+        gpu.set_tuning(kernel=kernel, early=0)
+        got = gpu.render_hdr(scene, opts).view(np.uint32)
+        assert gpu.last_kernel().endswith("(early)"), gpu.last_kernel()
+        assert np.array_equal(got, ref), f"kernel {kernel}"
+    gpu.set_tuning(kernel=0)
+    gpu.set_environment(None)
+    # Batched: the same two scenes in one launch, early and general.
+    dev = oxpt.Device(0)
+    pk = oxpt.Packed(scene, opts, env)
+    pk2 = oxpt.Packed(small_terrain(), opts)
+    ds = [dev.upload(pk), dev.upload(pk2)]
+    out = []
+    for early in (0, oxpt.EARLY_OFF):
+        dev.set_tuning(early=early)
+        acc = [dev.accum(opts.width, opts.height) for _ in ds]
+        dev.refine_batch([(s, a, s.params, 0, 5) for s, a in zip(ds, acc)])
+        dev.finish()
+        out.append([a.read().view(np.uint32).copy() for a in acc])
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
