@@ -71,7 +71,7 @@ def pmc_traffic(args, paths_per_launch):
     launch, each scaled by its reading on a plain copy of known size
     (profiles/counter_calibration_copy_1GiB.md). None for other configs."""
     path = os.path.join(ROOT, "profiles", "trace_persistent_c1_pmc.json")
-    kernel = "trace_persistent<4,early>"  # the default form for a scene without punctual lights
+    kernel = "trace_persistent<5,early>"  # the default form for a scene without punctual lights
     if args.workload != "c1" or args.kernel != 0 or args.spp or args.block or args.waves_per_cu or args.tail:
         return None
     if not os.path.exists(path):

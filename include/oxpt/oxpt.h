@@ -254,10 +254,11 @@ typedef struct oxpt_tuning {
     uint32_t block;          /* threads per workgroup: 64, 128, 256 */
     uint32_t waves_per_cu;   /* persistent residency: workgroups = CUs * this / waves per WG */
     uint32_t max_paths;      /* paths per launch cap (samples are split, order kept) */
-    uint32_t kernel;         /* 0 = default: oxpt_refine runs the persistent megakernel at 4 waves per SIMD
-                                (frames with fewer pixels than resident lanes: the one-job batched kernel),
-                                oxpt_refine_batch the batched (one path per work item) megakernel at 4
-                                (3 for rounds of fewer than 2^24 paths);
+    uint32_t kernel;         /* 0 = default: oxpt_refine runs the persistent megakernel at 4 waves per SIMD,
+                                5 for scenes without punctual lights (the early-continuation build; frames
+                                with fewer pixels than resident lanes: the one-job batched kernel),
+                                oxpt_refine_batch the batched (one path per work item) megakernel at 4,
+                                5 when no job has punctual lights (3 for rounds of fewer than 2^24 paths);
                                 1 = plain one-pixel-per-thread, 2..6 = persistent built for 3, 4, 5, 6, 8
                                 waves per SIMD, 7 = wavefront (separate shade / extend / occlude kernels),
                                 8 = persistent built for 2 waves per SIMD */

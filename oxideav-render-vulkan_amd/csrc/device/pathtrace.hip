@@ -916,9 +916,10 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
-    if (early && (kernel == 0 || kernel == 2 || kernel == 3 || kernel == 4)) {
+    if (early && (kernel == 0 || (kernel >= 2 && kernel <= 5))) {
         if (kernel == 2) hipLaunchKernelGGL((trace_persistent<3, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
         else if (kernel == 4) hipLaunchKernelGGL((trace_persistent<5, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
+        else if (kernel == 5) hipLaunchKernelGGL((trace_persistent<6, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
         else hipLaunchKernelGGL((trace_persistent<4, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
     } else switch (kernel) {
         case 2: hipLaunchKernelGGL(trace_persistent<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
@@ -943,9 +944,10 @@ hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t b
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
-    if (early && (kernel == 0 || kernel == 2 || kernel == 3 || kernel == 4)) {
+    if (early && (kernel == 0 || (kernel >= 2 && kernel <= 5))) {
         if (kernel == 2) hipLaunchKernelGGL((trace_batch<3, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
         else if (kernel == 4) hipLaunchKernelGGL((trace_batch<5, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
+        else if (kernel == 5) hipLaunchKernelGGL((trace_batch<6, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
         else hipLaunchKernelGGL((trace_batch<4, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
     } else switch (kernel) {
         case 2: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;

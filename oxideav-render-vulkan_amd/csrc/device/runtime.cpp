@@ -29,6 +29,9 @@ constexpr uint32_t kDefaultTail = 8;
 constexpr uint64_t kTailSlots = 1ull << 28;
 // Batched rounds of fewer paths than this run the 3-waves form by default.
 constexpr double kSmallRound = 16.0 * 1024 * 1024;
+// Default form of a refine without punctual lights (early continuation):
+// the persistent megakernel built for 5 waves per SIMD.
+constexpr uint32_t kEarlyDefaultForm = 4;
 
 // Waves per SIMD each megakernel form is built for (oxpt_tuning.kernel;
 // kernel 0 is 4 waves for both the single refine and the batched megakernel).
@@ -1012,7 +1015,13 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     uint64_t per = std::max<uint64_t>(1, c->tuning.max_paths / std::max<uint64_t>(pixels, 1));
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
-    const uint32_t lds = stack_lds_entries(c->tuning, block, false);
+    // Without punctual lights the early-continuation forms apply; their
+    // default runs at 5 waves per SIMD (fewer live registers than the
+    // general form, which keeps 4).
+    const bool early = c->tuning.early != OXPT_EARLY_OFF && P->cfg[2] == 0 && c->tuning.kernel != 1;
+    oxpt_tuning form = c->tuning;
+    if (form.kernel == 0 && early) form.kernel = kEarlyDefaultForm;
+    const uint32_t lds = stack_lds_entries(form, block, false);
     // Persistent forms: the last tail/16 of the frame's tiles are traced one
     // path per claim (a wave's lanes on one tile at one sample, so the wave
     // stays coherent), and lanes that run out of pixels take tail paths
@@ -1040,9 +1049,8 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
-        const bool early = c->tuning.early != OXPT_EARLY_OFF && args.P.cfg[2] == 0 && c->tuning.kernel != 1;
         c->last_kernel = c->tuning.kernel == 1 ? "trace_tiles" : early ? "trace_persistent(early)" : "trace_persistent";
-        hipError_t e = oxd::launch_trace(args, c->tuning.kernel, block, grid, lds, early, c->stream);
+        hipError_t e = oxd::launch_trace(args, form.kernel, block, grid, lds, early, c->stream);
         if (e != hipSuccess) return hip_fail(c, "trace launch", e);
         if (timed) {
             (void)hipEventRecord(ep.b, c->stream);
@@ -1195,8 +1203,11 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         // latency — weighs more than throughput.
         double round_paths = 0.0;
         for (double wk : work) round_paths += wk;
+        bool early = c->tuning.early != OXPT_EARLY_OFF;
+        for (const oxd::Job &J : sorted) early = early && J.P.cfg[2] == 0;
         oxpt_tuning form = c->tuning;
         if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = 2;
+        else if (form.kernel == 0 && early) form.kernel = kEarlyDefaultForm;
         const uint32_t lds = stack_lds_entries(form, block, true);
         hipError_t e = hipMemcpyAsync(c->d_jobs, sorted.data(), sorted.size() * sizeof(oxd::Job), hipMemcpyHostToDevice,
                                       c->stream);
@@ -1204,8 +1215,6 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
             e = hipMemcpyAsync(c->d_counters + c->job_cap, &nj, 4, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch job upload", e);
         oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap, c->d_samples};
-        bool early = c->tuning.early != OXPT_EARLY_OFF;
-        for (const oxd::Job &J : sorted) early = early && J.P.cfg[2] == 0;
         c->last_kernel = early ? "trace_batch(early)" : "trace_batch";
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
@@ -1274,8 +1283,8 @@ int oxpt_autotune(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, uint32
     struct Cand {
         uint32_t kernel, stack_lds, wf_variant, tail;
     };
-    const Cand cands[] = {{0, 0, 0, 0},  {0, 0, 0, 16}, {0, 0, 0, 4},  {2, 0, 0, 0},
-                          {4, 0, 0, 0},  {0, 8, 0, 0},  {0, OXPT_STACK_PRIVATE, 0, 0},
+    const Cand cands[] = {{0, 0, 0, 0}, {0, 0, 0, 16}, {0, 0, 0, 4}, {2, 0, 0, 0}, {3, 0, 0, 0},
+                          {4, 0, 0, 0}, {5, 0, 0, 0}, {0, 8, 0, 0}, {0, OXPT_STACK_PRIVATE, 0, 0},
                           {7, 0, 0, 0}};
     oxpt_accum *tmp = nullptr;
     if (int rc = oxpt_accum_create(c, P->dims[0], P->dims[1], &tmp)) return rc;
