@@ -174,9 +174,13 @@ def bench_serving(args):
 
     t = time.perf_counter()
     pool = workloads.uneven_batch(n=64, seed=3, threads=min(16, os.cpu_count() or 1), pack=False)
-    srv = oxpt.Server(0, hbm_budget=args.budget_gb << 30, spp_per_tick=args.spp_tick)
+    t_gen = time.perf_counter() - t
+    srv = oxpt.Server(0, hbm_budget=args.budget_gb << 30, spp_per_tick=args.spp_tick, device_build=args.device_build)
+    t = time.perf_counter()
     ids = [srv.add_scene(j.scene, j.env) for j in pool]
-    log(f"64 pool scenes generated + packed in {time.perf_counter() - t:.1f} s")
+    t_add = time.perf_counter() - t
+    log(f"64 pool scenes generated in {t_gen:.1f} s, registered in {t_add:.1f} s "
+        f"({'device' if args.device_build else 'host'} scene preparation)")
     reqs = workloads.serving_stream(duration=args.duration, rate=args.rate, pool=len(pool))
     n_render = sum(r.kind == "render" for r in reqs)
     scene_of = {r.seq: r.job for r in reqs if r.kind == "render"}
@@ -224,6 +228,8 @@ def bench_serving(args):
         "config": {"workload": f"serving: Poisson {args.rate} req/s for {args.duration:.0f} s from the uneven-batch "
                                f"distribution (64-scene pool) + refine(4) on 25% of renders; HBM budget "
                                f"{args.budget_gb} GB; {args.spp_tick} spp per tick",
+                   "scene_preparation": "device, at first use" if args.device_build else "host, at registration",
+                   "registration_s": round(t_add, 3),
                    "requests": len(handles), "failed": failed, "offered_paths": int(offered),
                    "offered_mpaths_s": round(offered / args.duration / 1e6, 3)},
         "latency_ms": {"p50": pct(50), "p99": pct(99)},
@@ -344,6 +350,7 @@ def main():
     ap.add_argument("--duration", type=float, default=60.0, help="c4: arrival window (s)")
     ap.add_argument("--budget-gb", type=int, default=200, help="c4: HBM admission budget")
     ap.add_argument("--spp-tick", type=int, default=16, help="c4: samples per request per tick")
+    ap.add_argument("--device-build", action="store_true", help="c4: prepare pool scenes on the device at first use")
     ap.add_argument("--drain-timeout", type=float, default=600.0, help="c4: wait per request (s)")
     ap.add_argument("--cpu-tiles", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg (PSNR, baseline, bytes/path)")

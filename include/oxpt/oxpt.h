@@ -515,6 +515,9 @@ typedef struct oxpt_server_config {
     uint32_t max_active;    /* request slots, 0 = unbounded */
     uint32_t keep_hdr;      /* also keep the RGBA32F result */
     uint32_t use_graph;     /* replay the batched step from a captured HIP graph */
+    uint32_t device_build;  /* keep pool scenes as host scenes and prepare them on the device at first
+                               use (oxpt_scene_build + device environment tables) instead of packing
+                               them on the host in oxpt_server_add_scene; same images */
 } oxpt_server_config;
 typedef struct oxpt_request_info {
     uint32_t ok, width, height, samples;

@@ -133,6 +133,14 @@ void bind(const Packed &pk, const EnvMap *env, const CameraFrame &cam, const oxp
 EnvWords env_layout(uint32_t w, uint32_t h);
 std::vector<double> env_row_weights(uint32_t h);
 
+// Device scene preparation (oxpt_scene_build) of a host scene: pk receives
+// the host-packed tables with the device-found emissive records and the
+// geometry counts Params need; geom / attrs / atlas stay on the device.
+int device_prepare(oxpt_ctx *ctx, const HostScene &s, Packed &pk, oxpt_scene **out, oxpt_build_info *info);
+// Upload the scene tables, with the environment section built on the
+// device after them when env is a full map (else the tables alone).
+int upload_tables(oxpt_ctx *ctx, oxpt_scene *dev, const std::vector<uint32_t> &scene_tables, const EnvMap *env);
+
 void sobol_directions(uint32_t out[OXPT_SOBOL_WORDS]);
 void sheen_table(float out[OXPT_SHEEN_WORDS]);
 uint16_t f32_to_f16(float v);

@@ -39,7 +39,10 @@ struct PoolScene {
     oxpt::Packed pk;
     std::optional<oxpt::EnvMap> env;
     oxpt::CameraSource cam;
-    std::vector<uint32_t> tables;  // with the environment appended
+    std::vector<uint32_t> tables;  // with the environment appended (host-packed scenes)
+    // Device-prepared scenes: the host scene, built on the device at each
+    // upload; upload_bytes is an upper bound until the first build.
+    std::optional<oxpt::HostScene> host;
     uint64_t upload_bytes = 0;
     oxpt_scene *dev = nullptr;
     uint64_t last_use = 0;
@@ -98,7 +101,24 @@ bool oxpt_server::admit_locked(Req &r) {
     if (!r.refine) need += (uint64_t)std::max(r.opts.width, 1u) * std::max(r.opts.height, 1u) * 16;
     if (resident + need > cfg.hbm_budget) evict_for_locked(need);
     if (resident + need > cfg.hbm_budget) return false;
-    if (!ps.dev) {
+    if (!ps.dev && ps.host) {
+        int rc = oxpt::device_prepare(ctx, *ps.host, ps.pk, &ps.dev, nullptr);
+        if (rc == OXPT_OK) {
+            rc = oxpt::upload_tables(ctx, ps.dev, ps.pk.tables, ps.env ? &*ps.env : nullptr);
+            if (rc != OXPT_OK) {
+                oxpt_scene_destroy(ps.dev);
+                ps.dev = nullptr;
+            }
+        }
+        if (rc != OXPT_OK) {
+            r.state = State::Rejected;
+            r.error = oxpt_last_error(ctx);
+            return true;
+        }
+        ps.upload_bytes = oxpt_scene_bytes(ps.dev);  // the actual size from now on
+        resident += ps.upload_bytes;
+        stats.uploads++;
+    } else if (!ps.dev) {
         oxpt_scene_desc d{};
         d.geom = ps.pk.geom.data();
         d.geom_vec4 = ps.pk.geom.size() / 4;
@@ -135,9 +155,10 @@ bool oxpt_server::admit_locked(Req &r) {
         }
         r.acc_bytes = (uint64_t)w * h * 16;
         resident += r.acc_bytes;
+        // Params only: the scene's tables (environment included) are resident.
         std::vector<uint32_t> tmp;
         oxpt::CameraFrame cam = oxpt::resolve_camera(ps.cam, w, h);
-        oxpt::bind(ps.pk, ps.env ? &*ps.env : nullptr, cam, r.opts, w, h, tmp, r.P);
+        oxpt::bind(ps.pk, ps.env ? &*ps.env : nullptr, cam, r.opts, w, h, tmp, r.P, false);
         r.first = 0;
         r.remaining = std::max(r.opts.samples_per_pixel, 1u);
         r.samples_total = r.remaining;
@@ -345,9 +366,27 @@ const char *oxpt_server_last_error(const oxpt_server *s) { return s ? s->err.c_s
 int oxpt_server_add_scene(oxpt_server *s, const oxpt_host_scene *scene, const oxpt_env_map *env, uint32_t *id) {
     if (!s || !scene || !id) return OXPT_E_ARG;
     auto ps = std::make_unique<PoolScene>();
-    ps->pk = oxpt::pack_scene(scene->s);
     if (env) ps->env = env->e;
     ps->cam = oxpt::camera_source(scene->s);
+    if (s->cfg.device_build && scene->s.soup.count() > 0) {
+        // Prepared on the device at first use. Admission bound: BVH nodes
+        // (at most 2 per triangle), 4-wide nodes, triangle and attribute
+        // words, tables and the environment section, the atlas.
+        ps->host = scene->s;
+        const oxpt::Packed layout = oxpt::pack_scene(scene->s, false, nullptr, false, false);
+        const uint64_t n = scene->s.soup.count();
+        const uint64_t env_words = env ? (uint64_t)(env->e.h + 1) + (uint64_t)env->e.h * (env->e.w + 1) +
+                                             4ull * env->e.w * env->e.h
+                                       : 0;
+        ps->upload_bytes = n * (64 + 64 + 48 + 192) + 4 * (layout.tables.size() + 4 * n + env_words) +
+                           (layout.chains.empty() ? 8 : (uint64_t)layout.atlas_side * layout.atlas_side *
+                                                            layout.atlas_layers * 8);
+        std::lock_guard<std::mutex> lk(s->mu);
+        *id = (uint32_t)s->pool.size();
+        s->pool.push_back(std::move(ps));
+        return OXPT_OK;
+    }
+    ps->pk = oxpt::pack_scene(scene->s);
     oxpt_render_options o;
     oxpt_render_options_default(&o);
     oxpt_params P;

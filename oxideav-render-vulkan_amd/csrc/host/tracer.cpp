@@ -71,54 +71,12 @@ struct oxpt_tracer {
     int upload_device(const oxpt::HostScene &s) {
         // Host: materials, lights, texture placement, Sobol' / sheen tables.
         // Device: lights, BVH, triangle / attribute words, 4-wide nodes, atlas.
-        auto pk = std::make_unique<oxpt::Packed>(oxpt::pack_scene(s, false, nullptr, false, false));
         if (dscene) {
             oxpt_scene_destroy(dscene);
             dscene = nullptr;
         }
-        const oxpt::Soup &so = s.soup;
-        const std::vector<float> mat_lum = oxpt::material_emission(s);
-        oxpt_soup_desc d{};
-        d.n_tris = so.count();
-        d.pos = so.pos.data();
-        d.normals = so.normals.empty() ? nullptr : so.normals.data();
-        d.tangents = so.tangents.empty() ? nullptr : so.tangents.data();
-        d.uv0 = so.uv0.empty() ? nullptr : so.uv0.data();
-        d.uv1 = so.uv1.empty() ? nullptr : so.uv1.data();
-        d.colors = so.colors.empty() ? nullptr : so.colors.data();
-        d.material = so.material.data();
-        d.flags = so.flags.data();
-        d.tables = pk->tables.data();
-        d.table_words = pk->tables.size();
-        d.atlas_side = pk->atlas_side;
-        d.atlas_layers = pk->atlas_layers;
-        d.material_emission = mat_lum.data();
-        d.n_materials = (uint32_t)mat_lum.size();
-        // Texels: decoded, filtered and converted on the device.
-        std::vector<oxpt_texture_desc> tex(s.textures.size());
-        for (size_t i = 0; i < s.textures.size(); i++)
-            tex[i] = {s.textures[i].w, s.textures[i].h, s.textures[i].rgba8.empty() ? nullptr : s.textures[i].rgba8.data()};
-        float lut[512];
-        oxpt::texel_decode_lut(lut);
-        oxpt_atlas_desc ad{tex.size(), tex.data(), pk->chains.size(), pk->chains.data(), lut};
-        d.atlas_build = pk->chains.empty() ? nullptr : &ad;
-        int rc = oxpt_scene_build(ctx, &d, nullptr, &dscene, &build_info);
-        if (rc) return dev_fail(rc);
-        // The tables now end with the emissive records found on the device.
-        size_t tw = 0;
-        rc = oxpt_scene_read(ctx, dscene, 2, nullptr, 0, &tw);
-        if (!rc) {
-            pk->tables.resize(tw);
-            rc = oxpt_scene_read(ctx, dscene, 2, pk->tables.data(), tw * 4, nullptr);
-        }
-        if (rc) return dev_fail(rc);
-        pk->n_emissive = build_info.n_emissive;
-        pk->off_elights = build_info.elight_offset;
-        pk->node_count = build_info.node_count;
-        pk->tri_base = build_info.tri_base;
-        pk->wide_base = build_info.wide_base;
-        pk->bvh_depth = build_info.bvh_depth;
-        pk->wide_depth = build_info.wide_depth;
+        auto pk = std::make_unique<oxpt::Packed>();
+        if (int rc = oxpt::device_prepare(ctx, s, *pk, &dscene, &build_info)) return dev_fail(rc);
         packed = std::move(pk);
         words_on_device = true;
         cam_src = oxpt::camera_source(s);
@@ -179,9 +137,7 @@ struct oxpt_tracer {
         int rc;
         if (env_dev) {
             // Environment sampling tables built on the device after the scene tables.
-            const std::vector<double> row_w = oxpt::env_row_weights(env->h);
-            oxpt_env_desc ed{env->w, env->h, env->rgb.data(), env->intensity, 0u, row_w.data()};
-            rc = oxpt_scene_set_tables_env(ctx, dscene, tables.data(), tables.size(), &ed);
+            rc = oxpt::upload_tables(ctx, dscene, tables, &*env);
             tables_on_device = true;
         } else {
             rc = oxpt_scene_set_tables(ctx, dscene, tables.data(), tables.size());

@@ -158,7 +158,7 @@ class BatchItem(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", u64), ("spp_per_tick", u32), ("max_active", u32), ("keep_hdr", u32),
-                ("use_graph", u32)]
+                ("use_graph", u32), ("device_build", u32)]
 
 
 class RequestInfo(C.Structure):
@@ -852,11 +852,12 @@ class Server:
     """Continuous-batching render server over one device (include/oxpt/oxpt.h, serving)."""
 
     def __init__(self, device=0, hbm_budget=200 << 30, spp_per_tick=16, max_active=0, keep_hdr=False,
-                 use_graph=False):
+                 use_graph=False, device_build=False):
         cfg = ServerConfig()
         lib.oxpt_server_config_default(C.byref(cfg))
         cfg.hbm_budget, cfg.spp_per_tick, cfg.max_active = hbm_budget, spp_per_tick, max_active
         cfg.keep_hdr, cfg.use_graph = 1 if keep_hdr else 0, 1 if use_graph else 0
+        cfg.device_build = 1 if device_build else 0
         h = _vp()
         rc = lib.oxpt_server_create(device, C.byref(cfg), C.byref(h))
         if rc:

@@ -15,10 +15,13 @@ def _pool(n=6, scale=0.01, seed=21):
     return workloads.uneven_batch(n=n, seed=seed, threads=8, scale=scale)
 
 
-@pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
-def test_served_images_equal_images_alone(use_graph):
+@pytest.mark.parametrize("use_graph,device_build", [(False, False), (True, False), (False, True)],
+                         ids=["eager", "graph", "device_build"])
+def test_served_images_equal_images_alone(use_graph, device_build):
+    """device_build: pool scenes are kept as host scenes and prepared on the
+    device at first use (BVH, words, lights, environment tables)."""
     jobs = _pool()
-    srv = oxpt.Server(0, spp_per_tick=4, keep_hdr=True, use_graph=use_graph)
+    srv = oxpt.Server(0, spp_per_tick=4, keep_hdr=True, use_graph=use_graph, device_build=device_build)
     ids = [srv.add_scene(j.scene, j.env) for j in jobs]
     reqs, refines = [], []
 
