@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (liboxpt_oracle.so). TEST INFRASTRUCTURE.
 
 Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
-import this module, and only as the checker or the timed CPU baseline.
+import this module, and only as the reference the GPU results are compared
+with or as the timed CPU baseline.
 """
 from __future__ import annotations
 

@@ -6,7 +6,8 @@
  * path-trace resolve src/shaders/resolve.wgsl:38-75,151-188), which in
  * turn mirrors the CPU `oxideav_render::PathTracer`. Only the tests, the
  * smoke check and the benchmark's CPU-baseline leg may load this library,
- * and only as the checker or the timed CPU baseline: the product path
+ * and only as the reference GPU results are compared with or as the timed
+ * CPU baseline: the product path
  * (liboxpt) never calls it.
  *
  * The oracle traces the same packed words the HIP kernels read

@@ -4,9 +4,9 @@ built on the GPU are the host packer's words bit for bit, and so the
 images are the same.
 
 The reference prepares its scene on the CPU (GpuPathTracer::upload,
-src/pathtrace.rs:970-1239, with oxideav-mesh3d's Bvh); the host packer is
-this build's restatement of that step, and its tree is pinned by the
-brute-force closest-hit tests in test_oracle.py.
+src/pathtrace.rs:970-1239, with oxideav-mesh3d's Bvh); the host packer
+restates that step, and its tree is pinned by the brute-force closest-hit
+tests in test_oracle.py.
 """
 import numpy as np
 import pytest
