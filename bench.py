@@ -72,7 +72,8 @@ def pmc_traffic(args, paths_per_launch):
     (profiles/counter_calibration_copy_1GiB.md). None for other configs."""
     path = os.path.join(ROOT, "profiles", "trace_persistent_c1_pmc.json")
     kernel = "trace_persistent<5,early>"  # the default form for a scene without punctual lights
-    if args.workload != "c1" or args.kernel != 0 or args.spp or args.block or args.waves_per_cu or args.tail:
+    if (args.workload != "c1" or args.kernel != 0 or args.spp or args.block or args.waves_per_cu or args.tail
+            or args.early_off):
         return None
     if not os.path.exists(path):
         return None
@@ -358,6 +359,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--wf-variant", type=int, default=0, help="wavefront kernel forms (kernel 7)")
+    ap.add_argument("--early-off", action="store_true",
+                    help="run the general megakernel forms (no early continuation; same images)")
     ap.add_argument("--tail", type=int, default=0, help="persistent kernel tail in 1/16 of the frame "
                     "(0 default, 65535 none)")
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (profiling runs)")
@@ -394,7 +397,7 @@ def main():
     pt = oxpt.GpuPathTracer(0)
     log("device:", pt.adapter_summary())
     pt.set_tuning(block=args.block, waves_per_cu=args.waves_per_cu, kernel=args.kernel, wf_variant=args.wf_variant,
-                  tail=args.tail)
+                  tail=args.tail, early=oxpt.EARLY_OFF if args.early_off else 0)
     if args.spp:
         opts.samples_per_pixel = args.spp
     t = time.perf_counter()

@@ -272,10 +272,13 @@ typedef struct oxpt_tuning {
                                 one path per claim (then added in sample order; at most 2^28 buffered
                                 samples per launch): 0 = default (8, half the frame), 1..16, or
                                 OXPT_TAIL_NONE (every pixel as one run of samples) */
-    uint32_t early;          /* scenes without punctual lights: 0 = default, the megakernel forms built to
-                                make the NEE shadow ray and the BSDF continuation together at a hit (the
-                                BSDF does not live across traversals); OXPT_EARLY_OFF = the general
-                                forms. Same images either way */
+    uint32_t early;          /* scenes with at most 31 punctual lights (forms 0 and 2..5): 0 = default,
+                                the megakernel forms built to make the NEE shadow rays and the BSDF
+                                continuation together at a hit (the BSDF does not live across
+                                traversals; with punctual lights the pending contributions wait in a
+                                per-lane queue in device memory, (3 * lights + 12) * 4 bytes per
+                                resident lane); OXPT_EARLY_OFF = the general forms. Same images
+                                either way */
 } oxpt_tuning;
 #define OXPT_EARLY_OFF 0xFFFFu
 #define OXPT_STACK_PRIVATE 0xFFFFu

@@ -104,10 +104,23 @@ struct PathState {
     bool front;
     float4 pl_u;
     uint32_t job;
-    // Early form (no punctual lights): the continuation ray's origin, sampled
-    // at the hit, and whether a continuation follows the NEE shadow ray.
+    // Early forms: the continuation ray's origin, sampled at the hit, and
+    // whether a continuation follows the NEE shadow rays; with punctual
+    // lights also the NEE jobs still to trace (bit j: job j) and the
+    // vertex's bounce (the rays' ids).
     F3 c_o;
     bool cont;
+    uint32_t mask, nee_k;
+};
+
+// Per-lane NEE queue of the early form with punctual lights, word-major
+// across the launch's lanes: punctual job j's pending contribution at words
+// 3j..3j+2, then the area / environment job's shadow ray (origin, direction,
+// t_max, ray id, skip) and its pending contribution.
+struct NeeQueue {
+    float *q;
+    uint32_t stride, lane;
+    __device__ __forceinline__ float &operator[](uint32_t w) const { return q[(size_t)w * stride + lane]; }
 };
 
 // Camera ray of sample `index` of pixel (x, y) (wgsl:1715-1764).
@@ -158,6 +171,8 @@ __device__ __forceinline__ void init_path(const oxpt_params &P, const uint32_t *
     st.job = 0;
     st.c_o = f3(0.0f);
     st.cont = false;
+    st.mask = 0;
+    st.nee_k = 0;
 }
 
 // The shadow ray of NEE job `job` at a surface vertex: punctual light
@@ -322,18 +337,60 @@ __device__ __forceinline__ bool bsdf_continue(const oxpt_params &P, const uint32
     return true;
 }
 
+// Early form with punctual lights: the next queued NEE shadow ray (job
+// order: punctual lights, then the area / environment job), else the
+// continuation. Returns true when the path ended.
+__device__ __forceinline__ bool next_queued(const Scene &S, PathState &st, const NeeQueue &Q, uint32_t n_punct) {
+    if (st.mask != 0) {
+        const uint32_t j = (uint32_t)__ffs(st.mask) - 1u;
+        st.mask &= st.mask - 1u;
+        if (j < n_punct) {
+            // The light's ray as nee_job made it (punctual_sample is exact).
+            const LightSample ls = punctual_sample(S, j, st.p);
+            st.r_o = spawn(st.p, st.ng, ls.l);
+            st.r_d = ls.l;
+            st.r_tmax = ls.distance >= 0.0f ? ls.distance * (1.0f - 1.0e-4f) : INF;
+            st.r_id = (st.nee_k << 16u) | 0x8000u | (j & 0x7fffu);
+            st.r_skip = NONE;
+            st.pending = F3{Q[3u * j], Q[3u * j + 1u], Q[3u * j + 2u]};
+        } else {
+            const uint32_t a = 3u * n_punct;
+            st.r_o = F3{Q[a], Q[a + 1u], Q[a + 2u]};
+            st.r_d = F3{Q[a + 3u], Q[a + 4u], Q[a + 5u]};
+            st.r_tmax = Q[a + 6u];
+            st.r_id = fbits(Q[a + 7u]);
+            st.r_skip = fbits(Q[a + 8u]);
+            st.pending = F3{Q[a + 9u], Q[a + 10u], Q[a + 11u]};
+        }
+        st.shadow = true;
+        return false;
+    }
+    if (!st.cont) return true;
+    st.shadow = false;
+    st.r_o = st.c_o;
+    st.r_d = st.d;
+    st.r_tmax = INF;
+    st.r_id = st.k << 16u;
+    st.r_skip = NONE;
+    return false;
+}
+
 // Advances the state machine past the ray just traced, given its hit
 // (wgsl:1766-2040). Returns true when the path ended; *covered says
 // whether the camera ray hit anything. Otherwise st.r_* holds the next ray.
 //
-// Early = true is the form for scenes without punctual lights (one NEE job
-// per vertex): the NEE shadow ray and the BSDF continuation are both made at
+// Mode 1 is the form for scenes without punctual lights (one NEE job per
+// vertex): the NEE shadow ray and the BSDF continuation are both made at
 // the hit, so the BSDF and the vertex need not live across a traversal; the
-// shadow ray is traced first, then the continuation. Every random number and
-// every sum is the same, so the images are bit-identical to the general form.
-template <bool Early>
+// shadow ray is traced first, then the continuation. Mode 2 does the same
+// with punctual lights: every NEE job is evaluated at the hit, its pending
+// contribution (and the area job's whole ray) goes to the lane's NEE queue,
+// and each punctual shadow ray is rebuilt from its light when traced. Every
+// random number and every sum is the same, in the same order, so the images
+// are bit-identical to the general form (mode 0).
+template <int Mode>
 __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol, PathState &st,
-                        const Hit &hit, bool *covered) {
+                        const Hit &hit, bool *covered, const NeeQueue &Q) {
     const uint32_t strategy = P.cfg[1];
     const bool has_env = P.env[2] != 0;
     const uint32_t n_punct = P.cfg[2];
@@ -341,7 +398,9 @@ __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *ld
     *covered = true;
     if (st.shadow) {
         if (hit.slot == NONE) st.radiance = st.radiance + st.pending;
-        if constexpr (Early) {
+        if constexpr (Mode == 2) {
+            return next_queued(S, st, Q, n_punct);
+        } else if constexpr (Mode == 1) {
             // The continuation was sampled at the hit; trace it now.
             if (!st.cont) return true;
             st.shadow = false;
@@ -411,7 +470,47 @@ __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *ld
             return true;
         }
         if (st.k >= P.dims[3]) return true;
-        if constexpr (Early) {
+        if constexpr (Mode == 2) {
+            const Bsdf bsdf = make_bsdf(S.lds_sheen, mat, -st.d, surf.ng, hit.front);
+            if (bsdf_black(bsdf)) return true;
+            const F3 p = surf.position, ng = bsdf.ng;
+            const float4 pl_u = pattern(lds_sobol, st.pixel_seed, st.index, 2u + 2u * st.k);
+            uint32_t mask = 0;
+            for (uint32_t j = 0; j <= n_punct; j++) {
+                const NeeRay nr = nee_job(P, S, bsdf, p, ng, pl_u, st.beta, st.k, j);
+                if (!nr.ok) continue;
+                mask |= 1u << j;
+                if (j < n_punct) {
+                    Q[3u * j] = nr.pending.x;
+                    Q[3u * j + 1u] = nr.pending.y;
+                    Q[3u * j + 2u] = nr.pending.z;
+                } else {
+                    const uint32_t a = 3u * n_punct;
+                    Q[a] = nr.o.x;
+                    Q[a + 1u] = nr.o.y;
+                    Q[a + 2u] = nr.o.z;
+                    Q[a + 3u] = nr.d.x;
+                    Q[a + 4u] = nr.d.y;
+                    Q[a + 5u] = nr.d.z;
+                    Q[a + 6u] = nr.tmax;
+                    Q[a + 7u] = bitsf(nr.id);
+                    Q[a + 8u] = bitsf(nr.skip);
+                    Q[a + 9u] = nr.pending.x;
+                    Q[a + 10u] = nr.pending.y;
+                    Q[a + 11u] = nr.pending.z;
+                }
+            }
+            st.nee_k = st.k;
+            F3 co = f3(0.0f);
+            st.cont = bsdf_continue(P, lds_sobol, S.lds_sheen, st, bsdf, p, ng, mat.thickness, mat.transmission,
+                                    mat.atten_color, mat.atten_dist, hit.front, &co);
+            st.c_o = co;
+            st.p = p;
+            st.ng = ng;
+            st.mask = mask;
+            return next_queued(S, st, Q, n_punct);
+        }
+        if constexpr (Mode == 1) {
             const Bsdf bsdf = make_bsdf(S.lds_sheen, mat, -st.d, surf.ng, hit.front);
             if (bsdf_black(bsdf)) return true;
             const F3 p = surf.position, ng = bsdf.ng;
@@ -487,13 +586,13 @@ __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *ld
 // One iteration: trace the current ray and advance the state machine.
 // `lds_stack` / `lds_depth`: this wave's LDS share of the traversal stack
 // (depth 0: the whole stack in private memory).
-template <bool Early = false>
+template <int Mode = 0>
 __device__ __forceinline__ bool step_path(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol,
                                           PathState &st, bool *covered, lds_u64 *lds_stack = nullptr,
-                                          uint32_t lds_depth = 0) {
+                                          uint32_t lds_depth = 0, const NeeQueue &Q = NeeQueue{nullptr, 0, 0}) {
     Hit hit = traverse<-1>(S, make_ray(st.r_o, st.r_d), st.r_tmax, st.path_seed, st.r_id, !st.shadow && st.k == 0,
                            st.shadow, st.r_skip, lds_stack, lds_depth);
-    return advance<Early>(P, S, lds_sobol, st, hit, covered);
+    return advance<Mode>(P, S, lds_sobol, st, hit, covered, Q);
 }
 
 __device__ __forceinline__ void stage_tables(const uint32_t *tables, const oxpt_params &P, uint32_t *lds_sobol,
@@ -515,7 +614,7 @@ __device__ __forceinline__ void item_pixel(uint32_t tiles_x, uint32_t item, uint
 // VGPRs, more latency hiding, possible spills): an autotuner knob.
 // lds_depth: traversal-stack entries per lane kept in dynamic LDS
 // (blockDim / 64 * lds_depth * 512 bytes), the rest in private memory.
-template <int MinWaves, bool Early = false>
+template <int MinWaves, int Mode = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves > 0 ? MinWaves : 1)))
 trace_persistent(TraceArgs A, uint32_t lds_depth) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
@@ -538,6 +637,7 @@ trace_persistent(TraceArgs A, uint32_t lds_depth) {
     bool pixels_spent = A.split == 0;
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     PathState st;
+    const NeeQueue Q{A.nee_q, gridDim.x * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     for (;;) {
         // Regenerate: idle lanes claim work with one atomic per wave — whole
         // pixels while items before the split remain, then single tail
@@ -599,7 +699,7 @@ trace_persistent(TraceArgs A, uint32_t lds_depth) {
         // before returning to the claim step, so the lanes of a wave stay in
         // the same phase of their bounces (less divergent shading).
         bool covered, ended;
-        while (!(ended = step_path<Early>(A.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) && st.shadow &&
+        while (!(ended = step_path<Mode>(A.P, S, lds_sobol, st, &covered, lds_stack, lds_depth, Q)) && st.shadow &&
                A.P.cfg[2] != 0) {
         }
         if (ended) {
@@ -677,7 +777,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) tr
 // launch's sample buffer; batch_reduce then adds every pixel's samples into
 // its accumulator in sample order, so every job's image equals its image
 // alone. Jobs are ordered largest first by the host.
-template <int MinWaves, bool Early = false>
+template <int MinWaves, int Mode = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves)))
 trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
     __shared__ uint32_t lds_sobol[OXPT_SOBOL_WORDS];
@@ -693,6 +793,7 @@ trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
     bool has_path = false;
     uint32_t item = 0;
     PathState st;
+    const NeeQueue Q{B.nee_q, gridDim.x * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     uint32_t j = 0;
     for (;;) {
         j = __builtin_amdgcn_readfirstlane(j);
@@ -736,7 +837,7 @@ trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
             }
             if (!has_path) continue;
             bool covered, ended;
-            while (!(ended = step_path<Early>(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth)) && st.shadow &&
+            while (!(ended = step_path<Mode>(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth, Q)) && st.shadow &&
                    J.P.cfg[2] != 0) {
             }
             if (ended) {
@@ -903,24 +1004,50 @@ namespace oxd {
 // Kernel forms (oxpt_tuning.kernel): 0 the default (4 waves per SIMD),
 // 1 plain per-pixel, 2..6 persistent built for 3, 4, 5, 6, 8 waves per SIMD,
 // 8 persistent built for 2 waves per SIMD (256 VGPRs).
-// early: the scene has no punctual lights, so the forms built with the
-// early continuation (3, 4 and 5 waves per SIMD) apply; same images.
+// mode 1 (no punctual lights) and mode 2 (1..31 punctual lights, NEE queue
+// in args.nee_q) select the early-continuation builds of forms 0 and 2..5
+// (3, 4, 5 and 6 waves per SIMD; form 0 is the 4-waves build); same images.
+template <int Mode>
+static void launch_persistent(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid,
+                              uint32_t lds_depth, size_t shm, hipStream_t s) {
+    switch (kernel) {
+        case 2: hipLaunchKernelGGL((trace_persistent<3, Mode>), dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 4: hipLaunchKernelGGL((trace_persistent<5, Mode>), dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        case 5: hipLaunchKernelGGL((trace_persistent<6, Mode>), dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+        default: hipLaunchKernelGGL((trace_persistent<4, Mode>), dim3(grid), dim3(block), shm, s, args, lds_depth); break;
+    }
+}
+
+template <int Mode>
+static void launch_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, uint32_t lds_depth,
+                         size_t shm, hipStream_t s) {
+    switch (kernel) {
+        case 2: hipLaunchKernelGGL((trace_batch<3, Mode>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        case 4: hipLaunchKernelGGL((trace_batch<5, Mode>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        case 5: hipLaunchKernelGGL((trace_batch<6, Mode>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+        default: hipLaunchKernelGGL((trace_batch<4, Mode>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
+    }
+}
+
+static bool early_form(uint32_t kernel, uint32_t mode) {
+    return (mode == 1 || mode == 2) && (kernel == 0 || (kernel >= 2 && kernel <= 5));
+}
+
 hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, uint32_t lds_depth,
-                        bool early, hipStream_t s) {
+                        uint32_t mode, hipStream_t s) {
     if (kernel == 1) {
         uint32_t blocks = args.n_items / 64u;
         hipLaunchKernelGGL(trace_tiles, dim3(blocks), dim3(64), 0, s, args);
         return hipGetLastError();
     }
+    if (early_form(kernel, mode) && mode == 2 && args.nee_q == nullptr) return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s, args.work);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
-    if (early && (kernel == 0 || (kernel >= 2 && kernel <= 5))) {
-        if (kernel == 2) hipLaunchKernelGGL((trace_persistent<3, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
-        else if (kernel == 4) hipLaunchKernelGGL((trace_persistent<5, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
-        else if (kernel == 5) hipLaunchKernelGGL((trace_persistent<6, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
-        else hipLaunchKernelGGL((trace_persistent<4, true>), dim3(grid), dim3(block), shm, s, args, lds_depth);
+    if (early_form(kernel, mode)) {
+        if (mode == 2) launch_persistent<2>(args, kernel, block, grid, lds_depth, shm, s);
+        else launch_persistent<1>(args, kernel, block, grid, lds_depth, shm, s);
     } else switch (kernel) {
         case 2: hipLaunchKernelGGL(trace_persistent<3>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
         case 4: hipLaunchKernelGGL(trace_persistent<5>, dim3(grid), dim3(block), shm, s, args, lds_depth); break;
@@ -939,16 +1066,15 @@ hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, 
 
 // Batched forms: the same numbering (default: the 4-waves build).
 hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid,
-                              uint32_t lds_depth, bool early, hipStream_t s) {
+                              uint32_t lds_depth, uint32_t mode, hipStream_t s) {
+    if (early_form(kernel, mode) && mode == 2 && args.nee_q == nullptr) return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, s, args.counters, args.n_jobs, args.max_jobs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t shm = (size_t)(block / 64u) * lds_depth * 64u * sizeof(uint64_t);
-    if (early && (kernel == 0 || (kernel >= 2 && kernel <= 5))) {
-        if (kernel == 2) hipLaunchKernelGGL((trace_batch<3, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
-        else if (kernel == 4) hipLaunchKernelGGL((trace_batch<5, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
-        else if (kernel == 5) hipLaunchKernelGGL((trace_batch<6, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
-        else hipLaunchKernelGGL((trace_batch<4, true>), dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth);
+    if (early_form(kernel, mode)) {
+        if (mode == 2) launch_batch<2>(args, kernel, block, grid, lds_depth, shm, s);
+        else launch_batch<1>(args, kernel, block, grid, lds_depth, shm, s);
     } else switch (kernel) {
         case 2: hipLaunchKernelGGL(trace_batch<3>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;
         case 4: hipLaunchKernelGGL(trace_batch<5>, dim3(grid), dim3(block), shm, s, args, args.jobs, lds_depth); break;

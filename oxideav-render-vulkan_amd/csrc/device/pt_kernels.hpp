@@ -27,6 +27,9 @@ struct TraceArgs {
     uint32_t split;
     uint32_t tail_work;     // (n_items - split) * count
     float4 *samples;
+    // Early form with punctual lights: each lane's NEE queue, word-major
+    // across the launch's lanes (word w of lane g at nee_q[w * lanes + g]).
+    float *nee_q;
 };
 
 // One job of a batched launch: a resident scene, its Params, an accumulator
@@ -54,16 +57,21 @@ struct BatchArgs {
     uint32_t *counters;  // one work counter per job
     uint32_t max_jobs;   // capacity of jobs / counters
     float4 *samples;     // one (rgb, covered) per traced path of the launch
+    float *nee_q;        // NEE queue of the early form with punctual lights (as TraceArgs)
 };
 
 // lds_depth: traversal-stack entries per lane held in LDS (dynamic shared
 // memory, block / 64 * lds_depth * 512 bytes per workgroup).
-// early: every job (scene) has no punctual lights; the forms built for that
-// case make the BSDF continuation at the hit (bit-identical images).
+// mode: 0 the general forms; 1 the early forms (no punctual lights: NEE
+// and BSDF continuation made together at a hit); 2 the early forms with a
+// per-lane NEE queue (1..31 punctual lights; nee_q holds
+// nee_queue_words(max punctual) words per lane). Same images in every mode.
+constexpr uint32_t kQueueMaxPunctual = 31;
+inline uint32_t nee_queue_words(uint32_t n_punct) { return 3u * n_punct + 12u; }
 hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid,
-                              uint32_t lds_depth, bool early, hipStream_t s);
+                              uint32_t lds_depth, uint32_t mode, hipStream_t s);
 hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, uint32_t lds_depth,
-                        bool early, hipStream_t s);
+                        uint32_t mode, hipStream_t s);
 hipError_t launch_resolve(const float4 *accum, uint32_t n, const oxpt_resolve_params &rp, void *out, hipStream_t s);
 hipError_t launch_debug_pattern(const uint32_t *tables, uint32_t off_sobol, uint32_t seed, uint32_t n,
                                 const uint32_t *x, const uint32_t *y, const uint32_t *index, const uint32_t *p,

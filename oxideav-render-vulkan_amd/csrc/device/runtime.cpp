@@ -32,6 +32,18 @@ constexpr double kSmallRound = 16.0 * 1024 * 1024;
 // Default form of a refine without punctual lights (early continuation):
 // the persistent megakernel built for 5 waves per SIMD.
 constexpr uint32_t kEarlyDefaultForm = 4;
+// Default form of a refine with 1..31 punctual lights (early continuation
+// with a per-lane NEE queue in device memory).
+constexpr uint32_t kQueueDefaultForm = 0;
+
+// Kernel mode of a refine whose scene(s) hold up to n_punct punctual lights
+// (0 general, 1 early, 2 early with an NEE queue).
+uint32_t trace_mode(const oxpt_tuning &t, uint32_t n_punct) {
+    // Early builds exist for the default form and forms 2..5.
+    if (t.early == OXPT_EARLY_OFF || !(t.kernel == 0 || (t.kernel >= 2 && t.kernel <= 5))) return 0;
+    if (n_punct == 0) return 1;
+    return n_punct <= oxd::kQueueMaxPunctual ? 2u : 0u;
+}
 
 // Waves per SIMD each megakernel form is built for (oxpt_tuning.kernel;
 // kernel 0 is 4 waves for both the single refine and the batched megakernel).
@@ -84,7 +96,9 @@ struct oxpt_ctx {
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;  // captured batched step (reset + trace)
     uint32_t graph_block = 0, graph_grid = 0, graph_kernel = 0, graph_lds = 0;
-    bool graph_early = false;
+    uint32_t graph_mode = 0;
+    float *d_nee_q = nullptr;  // NEE queue of the early form with punctual lights
+    size_t nee_q_bytes = 0;
     const char *last_kernel = "";  // name of the last trace kernel launched
     float4 *d_samples = nullptr;  // batched launch: one (rgb, covered) per path of a round
     size_t samples_bytes = 0;
@@ -506,6 +520,7 @@ void oxpt_ctx_destroy(oxpt_ctx *c) {
     dfree(c, c->d_jobs, c->job_cap * sizeof(oxd::Job));
     dfree(c, c->d_counters, c->job_cap * 4 + 4);
     dfree(c, c->d_samples, c->samples_bytes);
+    dfree(c, c->d_nee_q, c->nee_q_bytes);
     dfree(c, c->d_out, c->d_out_bytes);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -947,6 +962,26 @@ int oxpt_accum_write(oxpt_ctx *c, oxpt_accum *a, const float *host, size_t bytes
 // Grows the per-path sample buffer (batched launches and the persistent
 // kernel's tail) to at least `slots` float4s. A captured graph refers to the
 // old buffer, so it is dropped.
+// NEE queue of the early form with punctual lights: nee_queue_words(n_punct)
+// words for each lane of a launch of `lanes` threads.
+static int ensure_queue(oxpt_ctx *c, uint64_t lanes, uint32_t n_punct) {
+    const uint64_t bytes = lanes * oxd::nee_queue_words(n_punct) * 4u;
+    if (c->nee_q_bytes >= bytes) return OXPT_OK;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    dfree(c, c->d_nee_q, c->nee_q_bytes);
+    c->d_nee_q = nullptr;
+    c->nee_q_bytes = 0;
+    void *p = nullptr;
+    if (int rc = dalloc(c, bytes, &p, "NEE queue")) return rc;
+    c->d_nee_q = (float *)p;
+    c->nee_q_bytes = bytes;
+    return OXPT_OK;
+}
+
 static int ensure_samples(oxpt_ctx *c, uint64_t slots) {
     if (c->samples_bytes >= slots * 16) return OXPT_OK;
     (void)hipStreamSynchronize(c->stream);
@@ -1015,12 +1050,16 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     uint64_t per = std::max<uint64_t>(1, c->tuning.max_paths / std::max<uint64_t>(pixels, 1));
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
-    // Without punctual lights the early-continuation forms apply; their
-    // default runs at 5 waves per SIMD (fewer live registers than the
+    // The early-continuation forms apply to scenes with up to 31 punctual
+    // lights (beyond none, through the NEE queue); without punctual lights
+    // their default runs at 5 waves per SIMD (fewer live registers than the
     // general form, which keeps 4).
-    const bool early = c->tuning.early != OXPT_EARLY_OFF && P->cfg[2] == 0 && c->tuning.kernel != 1;
+    const uint32_t mode = trace_mode(c->tuning, P->cfg[2]);
     oxpt_tuning form = c->tuning;
-    if (form.kernel == 0 && early) form.kernel = kEarlyDefaultForm;
+    if (form.kernel == 0 && mode == 1) form.kernel = kEarlyDefaultForm;
+    if (form.kernel == 0 && mode == 2) form.kernel = kQueueDefaultForm;
+    if (mode == 2)
+        if (int rc = ensure_queue(c, (uint64_t)grid * block, P->cfg[2])) return rc;
     const uint32_t lds = stack_lds_entries(form, block, false);
     // Persistent forms: the last tail/16 of the frame's tiles are traced one
     // path per claim (a wave's lanes on one tile at one sample, so the wave
@@ -1049,8 +1088,12 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
-        c->last_kernel = c->tuning.kernel == 1 ? "trace_tiles" : early ? "trace_persistent(early)" : "trace_persistent";
-        hipError_t e = oxd::launch_trace(args, form.kernel, block, grid, lds, early, c->stream);
+        args.nee_q = c->d_nee_q;
+        c->last_kernel = c->tuning.kernel == 1 ? "trace_tiles"
+                         : mode == 2           ? "trace_persistent(early, queue)"
+                         : mode == 1           ? "trace_persistent(early)"
+                                               : "trace_persistent";
+        hipError_t e = oxd::launch_trace(args, form.kernel, block, grid, lds, mode, c->stream);
         if (e != hipSuccess) return hip_fail(c, "trace launch", e);
         if (timed) {
             (void)hipEventRecord(ep.b, c->stream);
@@ -1203,25 +1246,30 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         // latency — weighs more than throughput.
         double round_paths = 0.0;
         for (double wk : work) round_paths += wk;
-        bool early = c->tuning.early != OXPT_EARLY_OFF;
-        for (const oxd::Job &J : sorted) early = early && J.P.cfg[2] == 0;
+        uint32_t max_punct = 0;
+        for (const oxd::Job &J : sorted) max_punct = std::max(max_punct, J.P.cfg[2]);
+        const uint32_t mode = trace_mode(c->tuning, max_punct);
         oxpt_tuning form = c->tuning;
         if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = 2;
-        else if (form.kernel == 0 && early) form.kernel = kEarlyDefaultForm;
+        else if (form.kernel == 0 && mode == 1) form.kernel = kEarlyDefaultForm;
+        else if (form.kernel == 0 && mode == 2) form.kernel = kQueueDefaultForm;
+        if (mode == 2)
+            if (int rc = ensure_queue(c, (uint64_t)grid * block, max_punct)) return rc;
         const uint32_t lds = stack_lds_entries(form, block, true);
         hipError_t e = hipMemcpyAsync(c->d_jobs, sorted.data(), sorted.size() * sizeof(oxd::Job), hipMemcpyHostToDevice,
                                       c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(c->d_counters + c->job_cap, &nj, 4, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch job upload", e);
-        oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap, c->d_samples};
-        c->last_kernel = early ? "trace_batch(early)" : "trace_batch";
+        oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap, c->d_samples,
+                            c->d_nee_q};
+        c->last_kernel = mode == 2 ? "trace_batch(early, queue)" : mode == 1 ? "trace_batch(early)" : "trace_batch";
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
         if (c->use_graph) {
             if (c->graph && (c->graph_block != block || c->graph_grid != grid || c->graph_kernel != form.kernel ||
-                             c->graph_lds != lds || c->graph_early != early)) {
+                             c->graph_lds != lds || c->graph_mode != mode)) {
                 (void)hipGraphExecDestroy(c->graph);
                 c->graph = nullptr;
             }
@@ -1231,7 +1279,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
                 if (e == hipSuccess) {
-                    hipError_t le = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, early, c->stream);
+                    hipError_t le = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, c->stream);
                     e = hipStreamEndCapture(c->stream, &g);
                     if (e == hipSuccess) e = le;
                 }
@@ -1242,12 +1290,12 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 c->graph_grid = grid;
                 c->graph_kernel = form.kernel;
                 c->graph_lds = lds;
-                c->graph_early = early;
+                c->graph_mode = mode;
             }
             e = hipGraphLaunch(c->graph, c->stream);
             c->graph_launches++;
         } else {
-            e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, early, c->stream);
+            e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, c->stream);
         }
         if (e != hipSuccess) return hip_fail(c, "batch launch", e);
         if (timed) {

@@ -190,6 +190,47 @@ def test_punctual_lights_forms_bitwise(gpu):
     gpu.set_tuning(kernel=0)
 
 
+def test_punctual_queue_forms_bitwise(gpu):
+    """With 1..31 punctual lights the default forms evaluate every NEE job at
+    the hit, keep the pending contributions (and the area job's ray) in a
+    per-lane queue in device memory and make the BSDF continuation at once:
+    identical accumulators to the general forms — layered materials, 16
+    punctual lights + environment + emissive NEE, single refine and batched
+    (jobs with 16, 3 and no punctual lights in one launch). With more than
+    31 punctual lights the general form runs."""
+    scene = oxpt.Scene.layered(seed=5, n_tris=30000, n_textures=2, tex_size=64, n_lights=16)
+    env = oxpt.EnvironmentMap.sky(64, 32, 3)
+    opts = oxpt.RenderOptions(width=80, height=56, samples_per_pixel=5, max_bounces=10)
+    gpu.set_environment(env)
+    gpu.set_tuning(kernel=0, early=oxpt.EARLY_OFF)
+    ref = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+    assert "(early" not in gpu.last_kernel()
+    for kernel in (0, 2, 3, 4, 5):
+        gpu.set_tuning(kernel=kernel, early=0)
+        got = gpu.render_hdr(scene, opts).view(np.uint32)
+        assert gpu.last_kernel().endswith("(early, queue)"), gpu.last_kernel()
+        assert np.array_equal(got, ref), f"kernel {kernel}"
+    gpu.set_tuning(kernel=0)
+    many = oxpt.Scene.layered(seed=6, n_tris=8000, n_textures=1, tex_size=32, n_lights=40)
+    gpu.render_hdr(many, opts)
+    assert "(early" not in gpu.last_kernel(), gpu.last_kernel()
+    gpu.set_environment(None)
+    dev = oxpt.Device(0)
+    packs = [oxpt.Packed(scene, opts, env),
+             oxpt.Packed(oxpt.Scene.layered(seed=7, n_tris=12000, n_textures=1, tex_size=32, n_lights=3), opts),
+             oxpt.Packed(small_terrain(), opts)]
+    ds = [dev.upload(pk) for pk in packs]
+    out = []
+    for early in (0, oxpt.EARLY_OFF):
+        dev.set_tuning(early=early)
+        acc = [dev.accum(opts.width, opts.height) for _ in ds]
+        dev.refine_batch([(s, a, s.params, 0, 5) for s, a in zip(ds, acc)])
+        dev.finish()
+        out.append([a.read().view(np.uint32).copy() for a in acc])
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
 def test_kernel_forms_and_stack_split_bitwise(gpu):
     """Every occupancy form and every LDS/private split of the traversal stack
     (including a 1-entry LDS share, so deep pushes land in private memory)
