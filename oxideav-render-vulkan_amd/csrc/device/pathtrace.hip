@@ -590,8 +590,10 @@ template <int Mode = 0>
 __device__ __forceinline__ bool step_path(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol,
                                           PathState &st, bool *covered, lds_u64 *lds_stack = nullptr,
                                           uint32_t lds_depth = 0, const NeeQueue &Q = NeeQueue{nullptr, 0, 0}) {
-    Hit hit = traverse<-1>(S, make_ray(st.r_o, st.r_d), st.r_tmax, st.path_seed, st.r_id, !st.shadow && st.k == 0,
-                           st.shadow, st.r_skip, lds_stack, lds_depth);
+    // The NEE-queue form (punctual lights: most rays are shadow rays) walks
+    // any-hit queries without ordering the children.
+    Hit hit = traverse<-1, Mode == 2>(S, make_ray(st.r_o, st.r_d), st.r_tmax, st.path_seed, st.r_id,
+                                      !st.shadow && st.k == 0, st.shadow, st.r_skip, lds_stack, lds_depth);
     return advance<Mode>(P, S, lds_sobol, st, hit, covered, Q);
 }
 

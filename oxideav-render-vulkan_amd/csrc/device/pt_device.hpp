@@ -584,6 +584,10 @@ constexpr uint32_t WIDE_EMPTY = 0xffffffffu;
 // is tested, an inner node's hit children are ordered and the nearest
 // entered; when nothing is entered the stack is popped. Returns false when
 // the walk is over (stack empty, or an any-hit query answered).
+// AnyUnordered: any-hit queries skip the ordering (the answer is the same;
+// faster where shadow rays to punctual lights dominate, slower on open
+// scenes whose shadow rays find near occluders sooner in distance order).
+template <bool AnyUnordered = false>
 __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t_max, Hit &best, uint32_t &first,
                                           uint32_t &count, uint32_t &sp, uint32_t path_seed, uint32_t ray_id,
                                           bool primary, bool any, uint32_t skip, lds_u64 *lds, uint64_t *priv,
@@ -611,6 +615,31 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
             k[2] = t2 >= 0.0f ? ((uint64_t)(fbits(t2) & 0x7fffffffu) << 32) | 2u : ~0ull;
             k[3] = t3 >= 0.0f ? ((uint64_t)(fbits(t3) & 0x7fffffffu) << 32) | 3u : ~0ull;
         }
+        if (AnyUnordered && any) {
+            // Any-hit query: t_max stays put until the walk ends at the
+            // first accepted hit, so the order children are visited in
+            // cannot change the answer (is some triangle on the segment
+            // accepted) — only how soon it is found. Enter the first hit
+            // child in slot order and push the others unsorted.
+            bool entered = false;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (k[j] == ~0ull) continue;
+                const uint32_t cref = j == 0 ? rf.x : j == 1 ? rf.y : j == 2 ? rf.z : rf.w;
+                const uint32_t ccnt = j == 0 ? cn.x : j == 1 ? cn.y : j == 2 ? cn.z : cn.w;
+                if (!entered) {
+                    first = cref;
+                    count = ccnt;
+                    entered = true;
+                } else if (sp < WIDE_STACK) {
+                    const uint64_t e = (k[j] & 0xffffffff00000000ull) | (cref << 5 | ccnt);
+                    if (sp < in_lds) lds[sp * 64u] = e;
+                    else priv[sp - priv_off] = e;
+                    sp++;
+                }
+            }
+            if (entered) return true;
+        } else {
 #define OXD_CSWAP(a, b)                                 \
     {                                                   \
         const uint64_t lo_ = k[a] < k[b] ? k[a] : k[b]; \
@@ -640,6 +669,7 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
             count = c == 0 ? cn.x : c == 1 ? cn.y : c == 2 ? cn.z : cn.w;
             return true;
         }
+        }
     }
     while (sp != 0) {
         sp--;
@@ -653,7 +683,7 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
     return false;
 }
 
-template <int LdsDepth>
+template <int LdsDepth, bool AnyUnordered = false>
 __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                          uint32_t ray_id, bool primary, bool any, uint32_t skip, lds_u64 *lds_stack,
                                          uint32_t lds_depth) {
@@ -670,20 +700,21 @@ __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_m
     uint64_t priv[LdsDepth > 0 ? WIDE_STACK - LdsDepth : WIDE_STACK];
     constexpr uint32_t priv_off = LdsDepth > 0 ? (uint32_t)LdsDepth : 0u;
     uint32_t sp = 0, first = 0, count = 0;  // wide node 0, inner
-    while (wide_step(S, r, t_max, best, first, count, sp, path_seed, ray_id, primary, any, skip, lds, priv, in_lds,
-                     priv_off)) {
+    while (wide_step<AnyUnordered>(S, r, t_max, best, first, count, sp, path_seed, ray_id, primary, any, skip, lds,
+                                   priv, in_lds, priv_off)) {
     }
     return best;
 }
 
 // Closest (or, with `any`, some) accepted hit of the ray up to t_max_in:
 // the 4-wide walk when the scene carries one, else the binary walk.
-template <int LdsDepth = 0>
+template <int LdsDepth = 0, bool AnyUnordered = false>
 __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                         uint32_t ray_id, bool primary, bool any, uint32_t skip,
                                         lds_u64 *lds_stack = nullptr, uint32_t lds_depth = 0) {
     if (S.wide_base != 0)
-        return traverse4<LdsDepth>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack, lds_depth);
+        return traverse4<LdsDepth, AnyUnordered>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack,
+                                                 lds_depth);
     return traverse2<LdsDepth>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack, lds_depth);
 }
 
