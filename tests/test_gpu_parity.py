@@ -221,14 +221,39 @@ def test_punctual_queue_forms_bitwise(gpu):
              oxpt.Packed(small_terrain(), opts)]
     ds = [dev.upload(pk) for pk in packs]
     out = []
-    for early in (0, oxpt.EARLY_OFF):
+    # General forms, queue forms, and the queue forms replayed from a
+    # captured graph (twice: the second replay reuses the queue).
+    for early, graph in ((oxpt.EARLY_OFF, False), (0, False), (0, True), (0, True)):
         dev.set_tuning(early=early)
+        dev.set_graph(graph)
         acc = [dev.accum(opts.width, opts.height) for _ in ds]
         dev.refine_batch([(s, a, s.params, 0, 5) for s, a in zip(ds, acc)])
         dev.finish()
         out.append([a.read().view(np.uint32).copy() for a in acc])
-    for a, b in zip(out[0], out[1]):
-        assert np.array_equal(a, b)
+    dev.set_graph(False)
+    dev.set_tuning(early=0)
+    for run in out[1:]:
+        for a, b in zip(run, out[0]):
+            assert np.array_equal(a, b)
+
+
+def test_punctual_queue_limit(gpu):
+    """31 punctual lights fill the NEE queue's job mask (bit 31 is the area /
+    environment job): still the queue form, bitwise the general form; 32
+    lights run the general form."""
+    env = oxpt.EnvironmentMap.sky(64, 32, 3)
+    opts = oxpt.RenderOptions(width=64, height=48, samples_per_pixel=4, max_bounces=6)
+    gpu.set_environment(env)
+    for n_lights, queue in ((31, True), (32, False)):
+        scene = oxpt.Scene.layered(seed=11, n_tris=8000, n_textures=1, tex_size=32, n_lights=n_lights)
+        gpu.set_tuning(kernel=2, early=oxpt.EARLY_OFF)
+        ref = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+        gpu.set_tuning(kernel=2, early=0)
+        got = gpu.render_hdr(scene, opts).view(np.uint32)
+        assert gpu.last_kernel().endswith("(early, queue)") == queue, (n_lights, gpu.last_kernel())
+        assert np.array_equal(got, ref), n_lights
+    gpu.set_tuning(kernel=0)
+    gpu.set_environment(None)
 
 
 def test_kernel_forms_and_stack_split_bitwise(gpu):
