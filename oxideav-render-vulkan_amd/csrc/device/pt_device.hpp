@@ -580,10 +580,11 @@ __device__ __forceinline__ bool leaf_hits(const Scene &S, const Ray &r, uint32_t
 constexpr uint32_t WIDE_STACK = 96;
 constexpr uint32_t WIDE_EMPTY = 0xffffffffu;
 
-// One step of the 4-wide walk from the node or leaf (first, count): a leaf
-// is tested, an inner node's hit children are ordered and the nearest
-// entered; when nothing is entered the stack is popped. Returns false when
-// the walk is over (stack empty, or an any-hit query answered).
+// One step of the 4-wide walk from the node or leaf (first, count): a
+// leaf's next triangle is tested, an inner node's hit children are ordered
+// and the nearest entered; when a leaf is done or nothing is entered the
+// stack is popped. Returns false when the walk is over (stack empty, or an
+// any-hit query answered).
 // AnyUnordered: any-hit queries skip the ordering (the answer is the same;
 // faster where shadow rays to punctual lights dominate, slower on open
 // scenes whose shadow rays find near occluders sooner in distance order).
@@ -593,7 +594,13 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
                                           bool primary, bool any, uint32_t skip, lds_u64 *lds, uint64_t *priv,
                                           uint32_t in_lds, uint32_t priv_off) {
     if (count > 0) {
-        if (leaf_hits(S, r, first, count, t_max, best, path_seed, ray_id, primary, any, skip)) return false;
+        // One triangle per step (the leaf's triangles in slot order, as
+        // leaf_hits takes them): a lane at a leaf costs its wave one
+        // triangle test per step rather than the longest leaf of the wave.
+        if (leaf_hits(S, r, first, 1, t_max, best, path_seed, ray_id, primary, any, skip)) return false;
+        first++;
+        count--;
+        if (count > 0) return true;
     } else {
         const gf4_ptr nd = S.geom + S.wide_base + 8u * first;
         const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
