@@ -582,9 +582,10 @@ constexpr uint32_t WIDE_EMPTY = 0xffffffffu;
 
 // One step of the 4-wide walk from the node or leaf (first, count): a
 // leaf's next triangle is tested, an inner node's hit children are ordered
-// and the nearest entered; when a leaf is done or nothing is entered the
-// stack is popped. Returns false when the walk is over (stack empty, or an
-// any-hit query answered).
+// and the nearest entered; when a leaf is done, nothing is entered, or
+// there is no current node (first = WIDE_EMPTY), one stack entry is popped.
+// Returns false when the walk is over (stack empty, or an any-hit query
+// answered).
 // AnyUnordered: any-hit queries skip the ordering (the answer is the same;
 // faster where shadow rays to punctual lights dominate, slower on open
 // scenes whose shadow rays find near occluders sooner in distance order).
@@ -601,7 +602,7 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
         first++;
         count--;
         if (count > 0) return true;
-    } else {
+    } else if (first != WIDE_EMPTY) {
         const gf4_ptr nd = S.geom + S.wide_base + 8u * first;
         const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
         const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
@@ -678,16 +679,20 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
         }
         }
     }
-    while (sp != 0) {
-        sp--;
-        const uint64_t e = sp < in_lds ? lds[sp * 64u] : priv[sp - priv_off];
-        if (bitsf((uint32_t)(e >> 32)) <= t_max) {
-            first = (uint32_t)e >> 5;
-            count = (uint32_t)e & 31u;
-            return true;
-        }
+    // One pop per step as well: an entry beyond t_max leaves the lane with
+    // no current node (first = WIDE_EMPTY), and the next step pops again,
+    // so a lane discarding entries does not hold its wave in a pop loop.
+    if (sp == 0) return false;
+    sp--;
+    const uint64_t e = sp < in_lds ? lds[sp * 64u] : priv[sp - priv_off];
+    if (bitsf((uint32_t)(e >> 32)) <= t_max) {
+        first = (uint32_t)e >> 5;
+        count = (uint32_t)e & 31u;
+    } else {
+        first = WIDE_EMPTY;
+        count = 0;
     }
-    return false;
+    return true;
 }
 
 template <int LdsDepth, bool AnyUnordered = false>
