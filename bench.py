@@ -65,16 +65,23 @@ def workload(name):
     return scene, env, opts, desc
 
 
+# Committed PMC readings per workload: file and the default form they were taken on.
+PMC_FILES = {
+    "c1": ("trace_persistent_c1_pmc.json", "trace_persistent<5,early>"),  # no punctual lights
+    "c2": ("trace_persistent_c2_pmc.json", "trace_persistent<4,queue>"),  # 16 punctual lights
+}
+
+
 def pmc_traffic(args, paths_per_launch):
-    """HBM bytes per launch of the default kernel on the default workload,
-    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE readings of that
-    launch, each scaled by its reading on a plain copy of known size
+    """HBM bytes per launch of the default kernel on the single-scene
+    workloads, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE readings
+    of that launch, each scaled by its reading on a plain copy of known size
     (profiles/counter_calibration_copy_1GiB.md). None for other configs."""
-    path = os.path.join(ROOT, "profiles", "trace_persistent_c1_pmc.json")
-    kernel = "trace_persistent<5,early>"  # the default form for a scene without punctual lights
-    if (args.workload != "c1" or args.kernel != 0 or args.spp or args.block or args.waves_per_cu or args.tail
-            or args.early_off):
+    if (args.workload not in PMC_FILES or args.kernel != 0 or args.spp or args.block or args.waves_per_cu
+            or args.tail or args.early_off):
         return None
+    name, kernel = PMC_FILES[args.workload]
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
