@@ -214,13 +214,26 @@ __device__ __forceinline__ Level tex_level(const Scene &S, uint32_t desc, uint32
 __device__ __forceinline__ float h2f(uint32_t bits) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)bits);
 }
+// i mod n in [0, n), exactly, for n >= 1 and |i / n| <= 2^21 (texel
+// indices: |u| <= 1e6 times the level size). The float quotient is within
+// 1 of floor(i / n) there (|i| rounds to float with relative error 2^-24,
+// the fast divide adds 2 ulp: < 0.5 in all), and one correction step
+// makes the integer remainder exact — the value ((i % n) + n) % n gives,
+// without two integer divisions.
+__device__ __forceinline__ int32_t pos_mod(int32_t i, int32_t n) {
+    const float q = floorf(__fdividef((float)i, (float)n));
+    int32_t r = i - (int32_t)q * n;
+    r = r < 0 ? r + n : r;
+    r = r >= n ? r - n : r;
+    return r;
+}
 __device__ __forceinline__ int32_t wrap_index(int32_t i, int32_t n, uint32_t mode) {
     if (mode == 0) return min(max(i, 0), n - 1);
     if (mode == 1) {
-        int32_t m = ((i % (2 * n)) + 2 * n) % (2 * n);
+        int32_t m = pos_mod(i, 2 * n);
         return m >= n ? 2 * n - 1 - m : m;
     }
-    return ((i % n) + n) % n;
+    return pos_mod(i, n);
 }
 __device__ __forceinline__ float4 fetch(const Scene &S, const Level &lv, int32_t x, int32_t y, uint32_t ws, uint32_t wt) {
     int32_t xi = wrap_index(x, lv.w, ws), yi = wrap_index(y, lv.h, wt);
