@@ -818,10 +818,11 @@ trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
                     if (it >= n_paths) {
                         dry = true;
                     } else {
-                        const uint32_t tile = it / per_tile, rem = it - tile * per_tile;
+                        const uint32_t tile = udiv(it, J.tile_m, J.tile_l), rem = it - tile * per_tile;
                         const uint32_t s = rem >> 6, in = rem & 63u;
-                        const uint32_t px = (tile % J.tiles_x) * 8u + (in & 7u);
-                        const uint32_t py = (tile / J.tiles_x) * 8u + (in >> 3);
+                        const uint32_t ty = udiv(tile, J.tx_m, J.tx_l), tx = tile - ty * J.tiles_x;
+                        const uint32_t px = tx * 8u + (in & 7u);
+                        const uint32_t py = ty * 8u + (in >> 3);
                         // Items outside the image (ragged edge tiles) have
                         // no path; the reduce never reads them.
                         if (px < w && py < h) {

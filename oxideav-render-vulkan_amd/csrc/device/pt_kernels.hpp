@@ -44,9 +44,31 @@ struct alignas(16) Job {
     float4 *accum;
     uint32_t atlas_side, first, count, tiles_x, n_items;
     uint32_t sample_base;
-    uint32_t pad[4];
+    // The claim step's divisors as multiply-shift pairs (udiv_magic): the
+    // paths per tile (64 * count) and tiles_x.
+    uint32_t tile_m, tile_l, tx_m, tx_l;
 };
 static_assert(sizeof(Job) == 256, "Job is 256 bytes");
+
+// Unsigned division by a divisor known ahead, exact for every 32-bit n
+// (the round-up multiply-shift method with an add step, Granlund and
+// Montgomery 1994, as in Hacker's Delight 10-8): l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1; q = (t + ((n - t) >> 1)) >> (l - 1)
+// with t = the high word of n * m. d = 1 is l = 0 (q = n).
+struct UDiv {
+    uint32_t m, l;
+};
+inline UDiv udiv_magic(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < d) l++;
+    if (l == 0) return {0u, 0u};
+    return {(uint32_t)(((((1ull << l) - d) << 32) / d) + 1ull), l};
+}
+__device__ __forceinline__ uint32_t udiv(uint32_t n, uint32_t m, uint32_t l) {
+    if (l == 0) return n;
+    const uint32_t t = __umulhi(n, m);
+    return (t + ((n - t) >> 1)) >> (l - 1u);
+}
 
 // The job count lives in device memory (written with the job table), so
 // the launch arguments never change and the step can be captured once in a

@@ -1226,6 +1226,11 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
             uint32_t tx = (it.accum->w + 7u) / 8u, ty = (it.accum->h + 7u) / 8u;
             J.tiles_x = tx;
             J.n_items = 64u * tx * ty;
+            const oxd::UDiv dt = oxd::udiv_magic(64u * J.count), dx = oxd::udiv_magic(tx);
+            J.tile_m = dt.m;
+            J.tile_l = dt.l;
+            J.tx_m = dx.m;
+            J.tx_l = dx.l;
             jobs.push_back(J);
             work.push_back((double)it.accum->w * it.accum->h * J.count);
         }
