@@ -307,6 +307,18 @@ int oxpt_debug_light_picks(oxpt_ctx *ctx, const oxpt_scene *scene, const oxpt_pa
 /* Closest hit of n rays (o, d float3 each): {slot, t bits}. */
 int oxpt_debug_closest_hit(oxpt_ctx *ctx, const oxpt_scene *scene, const oxpt_params *P, size_t n,
                            const float *o3, const float *d3, uint32_t *out2);
+/* The batched claim step's exact unsigned divisions (no counterpart in the
+ * reference, which divides with WGSL's `/`): the host's multiply-shift pair
+ * for divisor d (host only, no device), and the device quotient num[i] /
+ * den[i] computed with it (den[i] != 0). */
+void oxpt_udiv_magic(uint32_t d, uint32_t *m, uint32_t *l);
+int oxpt_debug_udiv(oxpt_ctx *ctx, size_t n, const uint32_t *num, const uint32_t *den, uint32_t *out);
+/* Texel indices of one texture coordinate u on a level of `size` texels
+ * (1..65535) with wrap mode 0 clamp / 1 mirror / 2 repeat, as the kernels'
+ * filter_level forms them (pathtrace.wgsl:335-380): {nearest, bilinear x0,
+ * bilinear x0 + 1}, each wrapped. */
+int oxpt_debug_texel_index(oxpt_ctx *ctx, size_t n, const float *u, const int32_t *size, const uint32_t *mode,
+                           int32_t *out3);
 
 /* =====================================================================
  * Layer 2: host tracer and scenes.

@@ -66,6 +66,8 @@ def _load():
     sig("oxo_bsdf_probe", None, P(u32), P(f32), P(f32), P(f32), C.c_int, P(f32), P(f32), P(f32))
     sig("oxo_solid_angle", f32, P(f32), P(f32), P(f32), P(f32), C.c_int)
     sig("oxo_sample_spherical_triangle", None, P(f32), P(f32), P(f32), P(f32), f32, f32, C.c_int, P(f32))
+    sig("oxo_texel_index", None, f32, C.c_int32, u32, P(C.c_int32))
+    sig("oxo_ndf", f32, C.c_int, f32, f32)
     return lib
 
 
@@ -187,3 +189,15 @@ def sample_spherical_triangle(p, a, b, c, u0, u1, f64=False) -> np.ndarray:
     out = (f32 * 3)()
     lib.oxo_sample_spherical_triangle(_v3(p), _v3(a), _v3(b), _v3(c), u0, u1, 1 if f64 else 0, out)
     return np.array(list(out), dtype=np.float32)
+
+
+def texel_index(u: float, size: int, mode: int) -> tuple:
+    """{nearest, bilinear x0, x0 + 1} texel indices of coordinate u, wrapped."""
+    out = (C.c_int32 * 3)()
+    lib.oxo_texel_index(float(u), int(size), int(mode), out)
+    return tuple(out)
+
+
+def ndf(kind: int, n_dot_h: float, a: float) -> float:
+    """kind 0: GGX D; kind 1: Charlie D."""
+    return lib.oxo_ndf(kind, float(n_dot_h), float(a))

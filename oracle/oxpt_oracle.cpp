@@ -233,6 +233,7 @@ Level tex_level(const Ctx &c, uint32_t desc, uint32_t level) {
             (int32_t)(wh & 0xffffu), (int32_t)(wh >> 16u)};
 }
 
+// wrap_index (wgsl:335-347): clamp / mirrored repeat / repeat.
 int32_t wrap_index(int32_t i, int32_t n, uint32_t mode) {
     if (mode == 0) return std::min(std::max(i, 0), n - 1);
     if (mode == 1) {
@@ -242,6 +243,16 @@ int32_t wrap_index(int32_t i, int32_t n, uint32_t mode) {
     }
     return ((i % n) + n) % n;
 }
+
+// i32(floor(x)) as the reference converts it (WGSL conversion rules, the CPU
+// tracer's `as i32`): saturating at the i32 range. x is finite.
+int32_t sat_i32(float f) {
+    if (f >= 2147483648.0f) return INT32_MAX;
+    if (f < -2147483648.0f) return INT32_MIN;
+    return (int32_t)f;
+}
+// i + 1 in wrapping i32 arithmetic (WGSL integer addition).
+int32_t next_i32(int32_t i) { return (int32_t)((uint32_t)i + 1u); }
 
 float half_to_float(uint16_t h) {
     uint32_t s = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
@@ -270,13 +281,14 @@ V4 filter_level(const Ctx &c, const Level &lv, V2 uv, bool linear, uint32_t ws, 
     u = clampf(u, -1.0e6f, 1.0e6f);
     v = clampf(v, -1.0e6f, 1.0e6f);
     float x = u * (float)lv.w, y = v * (float)lv.h;
-    if (!linear) return fetch(c, lv, (int32_t)std::floor(x), (int32_t)std::floor(y), ws, wt);
+    if (!linear) return fetch(c, lv, sat_i32(std::floor(x)), sat_i32(std::floor(y)), ws, wt);
     float xf = x - 0.5f, yf = y - 0.5f;
     float x0f = std::floor(xf), y0f = std::floor(yf);
     float fx = xf - x0f, fy = yf - y0f;
-    int32_t x0 = (int32_t)x0f, y0 = (int32_t)y0f;
+    int32_t x0 = sat_i32(x0f), y0 = sat_i32(y0f);
     V4 taps[4];
-    for (int t = 0; t < 4; t++) taps[t] = fetch(c, lv, x0 + (t & 1), y0 + (t >> 1), ws, wt);
+    for (int t = 0; t < 4; t++)
+        taps[t] = fetch(c, lv, (t & 1) ? next_i32(x0) : x0, (t >> 1) ? next_i32(y0) : y0, ws, wt);
     return mix4(mix4(taps[0], taps[1], fx), mix4(taps[2], taps[3], fx), fy);
 }
 
@@ -1947,6 +1959,19 @@ float oxo_solid_angle(const float p[3], const float a[3], const float b[3], cons
     V3 P = {p[0], p[1], p[2]}, A = {a[0], a[1], a[2]}, B = {b[0], b[1], b[2]}, C = {c[0], c[1], c[2]};
     return f64 ? triangle_solid_angle_f64(P, A, B, C) : triangle_solid_angle_f32(P, A, B, C);
 }
+
+void oxo_texel_index(float u, int32_t size, uint32_t mode, int32_t out[3]) {
+    // The index steps of filter_level (wgsl:355-380) for one coordinate.
+    if (!finite(u)) u = 0.0f;
+    u = clampf(u, -1.0e6f, 1.0e6f);
+    float x = u * (float)size;
+    int32_t x0 = sat_i32(std::floor(x - 0.5f));
+    out[0] = wrap_index(sat_i32(std::floor(x)), size, mode);
+    out[1] = wrap_index(x0, size, mode);
+    out[2] = wrap_index(next_i32(x0), size, mode);
+}
+
+float oxo_ndf(int kind, float n_dot_h, float a) { return kind == 0 ? d_ggx(n_dot_h, a) : d_charlie(n_dot_h, a); }
 
 void oxo_sample_spherical_triangle(const float p[3], const float a[3], const float b[3], const float c[3],
                                    float u0, float u1, int f64, float out[3]) {

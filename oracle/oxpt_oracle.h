@@ -110,6 +110,13 @@ void oxo_bsdf_probe(const uint32_t *mat, const float sheen_lut[OXPT_SHEEN_WORDS]
                     const float n[3], const float v[3], int front, const float u[3],
                     const float l[3], float out[13]);
 
+/* Texel indices of one coordinate u on a level of `size` texels, wrap mode
+ * 0 clamp / 1 mirror / 2 repeat: {nearest, bilinear x0, x0 + 1}, wrapped. */
+void oxo_texel_index(float u, int32_t size, uint32_t mode, int32_t out[3]);
+/* Microfacet distributions: kind 0 GGX D(n.h; a) (wgsl:922-930), kind 1
+ * Charlie D(n.h; a) (wgsl:984-991). */
+float oxo_ndf(int kind, float n_dot_h, float a);
+
 /* Spherical-triangle probes: solid angle and an Arvo sample (f32 or f64). */
 float oxo_solid_angle(const float p[3], const float a[3], const float b[3], const float c[3],
                       int f64);

@@ -979,6 +979,28 @@ __global__ void debug_pattern_words(const uint32_t *__restrict__ tables, uint32_
     out[i] = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
+// The claim step's exact divisions (udiv with the host's udiv_magic pairs).
+__global__ void debug_udiv(uint32_t n, const uint32_t *num, const uint32_t *m, const uint32_t *l, uint32_t *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = udiv(num[i], m[i], l[i]);
+}
+
+// Texel indices of one coordinate as filter_level forms them: the nearest
+// tap, and the bilinear footprint's two taps, wrapped by the sampler mode.
+__global__ void debug_texel_index(uint32_t n, const float *u, const int32_t *size, const uint32_t *mode, int32_t *out3) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = u[i];
+    if (!finite(v)) v = 0.0f;
+    v = clampf(v, -1.0e6f, 1.0e6f);
+    const int32_t w = size[i];
+    const float x = v * (float)w;
+    const int32_t x0 = sat_i32(floorf(x - 0.5f));
+    out3[3 * i] = wrap_index(sat_i32(floorf(x)), w, mode[i]);
+    out3[3 * i + 1] = wrap_index(x0, w, mode[i]);
+    out3[3 * i + 2] = wrap_index(next_i32(x0), w, mode[i]);
+}
+
 __global__ void debug_light_picks(TraceArgs A, uint32_t n, const float *u, uint32_t *out3) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1113,6 +1135,22 @@ hipError_t launch_debug_pattern(const uint32_t *tables, uint32_t off_sobol, uint
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(debug_pattern_words, dim3(blocks), dim3(256), 0, s, tables, off_sobol, seed, n, x, y, index, p,
                        reinterpret_cast<uint4 *>(out4));
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_udiv(uint32_t n, const uint32_t *num, const uint32_t *m, const uint32_t *l, uint32_t *out,
+                             hipStream_t s) {
+    uint32_t blocks = (n + 255u) / 256u;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_udiv, dim3(blocks), dim3(256), 0, s, n, num, m, l, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_texel_index(uint32_t n, const float *u, const int32_t *size, const uint32_t *mode,
+                                    int32_t *out3, hipStream_t s) {
+    uint32_t blocks = (n + 255u) / 256u;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_texel_index, dim3(blocks), dim3(256), 0, s, n, u, size, mode, out3);
     return hipGetLastError();
 }
 

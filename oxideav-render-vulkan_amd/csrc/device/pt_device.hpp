@@ -222,11 +222,20 @@ __device__ __forceinline__ float h2f(uint32_t bits) {
 // without two integer divisions.
 __device__ __forceinline__ int32_t pos_mod(int32_t i, int32_t n) {
     const float q = floorf(__fdividef((float)i, (float)n));
-    int32_t r = i - (int32_t)q * n;
+    // i - q n in wrapping 32-bit arithmetic: the true remainder is within
+    // (-n, 2n), so the wrapped difference is that remainder.
+    int32_t r = (int32_t)((uint32_t)i - (uint32_t)(int32_t)q * (uint32_t)n);
     r = r < 0 ? r + n : r;
     r = r >= n ? r - n : r;
     return r;
 }
+// The reference's i32(f) of a floored texel coordinate: saturating at the
+// i32 range (WGSL conversion rules; the CPU tracer's `as i32`). f is finite.
+__device__ __forceinline__ int32_t sat_i32(float f) {
+    return f >= 2147483648.0f ? 2147483647 : f < -2147483648.0f ? (-2147483647 - 1) : (int32_t)f;
+}
+// i + 1 in wrapping i32 arithmetic (the bilinear footprint's right / lower tap).
+__device__ __forceinline__ int32_t next_i32(int32_t i) { return (int32_t)((uint32_t)i + 1u); }
 __device__ __forceinline__ int32_t wrap_index(int32_t i, int32_t n, uint32_t mode) {
     if (mode == 0) return min(max(i, 0), n - 1);
     if (mode == 1) {
@@ -249,13 +258,14 @@ __device__ float4 filter_level(const Scene &S, const Level &lv, float2 uv, bool 
     u = clampf(u, -1.0e6f, 1.0e6f);
     v = clampf(v, -1.0e6f, 1.0e6f);
     float x = u * (float)lv.w, y = v * (float)lv.h;
-    if (!linear) return fetch(S, lv, (int32_t)floorf(x), (int32_t)floorf(y), ws, wt);
+    if (!linear) return fetch(S, lv, sat_i32(floorf(x)), sat_i32(floorf(y)), ws, wt);
     float xf = x - 0.5f, yf = y - 0.5f;
     float x0f = floorf(xf), y0f = floorf(yf);
     float fx = xf - x0f, fy = yf - y0f;
-    int32_t x0 = (int32_t)x0f, y0 = (int32_t)y0f;
-    float4 t0 = fetch(S, lv, x0, y0, ws, wt), t1 = fetch(S, lv, x0 + 1, y0, ws, wt);
-    float4 t2 = fetch(S, lv, x0, y0 + 1, ws, wt), t3 = fetch(S, lv, x0 + 1, y0 + 1, ws, wt);
+    int32_t x0 = sat_i32(x0f), y0 = sat_i32(y0f);
+    const int32_t x1 = next_i32(x0), y1 = next_i32(y0);
+    float4 t0 = fetch(S, lv, x0, y0, ws, wt), t1 = fetch(S, lv, x1, y0, ws, wt);
+    float4 t2 = fetch(S, lv, x0, y1, ws, wt), t3 = fetch(S, lv, x1, y1, ws, wt);
     return mix4(mix4(t0, t1, fx), mix4(t2, t3, fx), fy);
 }
 __device__ float4 sample_lod(const Scene &S, uint32_t desc, float2 uv, float lod) {

@@ -99,6 +99,10 @@ hipError_t launch_debug_pattern(const uint32_t *tables, uint32_t off_sobol, uint
                                 const uint32_t *x, const uint32_t *y, const uint32_t *index, const uint32_t *p,
                                 uint32_t *out4, hipStream_t s);
 hipError_t launch_debug_picks(const TraceArgs &args, uint32_t n, const float *u, uint32_t *out3, hipStream_t s);
+hipError_t launch_debug_udiv(uint32_t n, const uint32_t *num, const uint32_t *m, const uint32_t *l, uint32_t *out,
+                             hipStream_t s);
+hipError_t launch_debug_texel_index(uint32_t n, const float *u, const int32_t *size, const uint32_t *mode,
+                                    int32_t *out3, hipStream_t s);
 hipError_t launch_debug_hits(const TraceArgs &args, uint32_t n, const float *o3, const float *d3, uint32_t *out2,
                              hipStream_t s);
 

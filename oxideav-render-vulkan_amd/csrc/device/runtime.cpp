@@ -1485,6 +1485,61 @@ int oxpt_debug_light_picks(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *
     return e == hipSuccess ? OXPT_OK : hip_fail(c, "debug light picks", e);
 }
 
+void oxpt_udiv_magic(uint32_t d, uint32_t *m, uint32_t *l) {
+    const oxd::UDiv u = oxd::udiv_magic(d);
+    if (m) *m = u.m;
+    if (l) *l = u.l;
+}
+
+int oxpt_debug_udiv(oxpt_ctx *c, size_t n, const uint32_t *num, const uint32_t *den, uint32_t *out) {
+    if (!c || !num || !den || !out) return OXPT_E_ARG;
+    if (n == 0) return OXPT_OK;
+    if (n > (1u << 26)) return fail(c, OXPT_E_ARG, "debug: too many divisions");
+    for (size_t i = 0; i < n; i++)
+        if (den[i] == 0) return fail(c, OXPT_E_ARG, "debug: division by zero");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    std::vector<uint32_t> m(n), l(n);
+    for (size_t i = 0; i < n; i++) oxpt_udiv_magic(den[i], &m[i], &l[i]);
+    void *d = nullptr;
+    size_t bytes = n * 16;
+    if (int rc = dalloc(c, bytes, &d, "debug buffers")) return rc;
+    uint32_t *dn = (uint32_t *)d, *dm = dn + n, *dl = dm + n, *dout = dl + n;
+    hipError_t e = hipMemcpyAsync(dn, num, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dm, m.data(), n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dl, l.data(), n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = oxd::launch_debug_udiv((uint32_t)n, dn, dm, dl, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(c, d, bytes);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "debug udiv", e);
+}
+
+int oxpt_debug_texel_index(oxpt_ctx *c, size_t n, const float *u, const int32_t *size, const uint32_t *mode,
+                           int32_t *out3) {
+    if (!c || !u || !size || !mode || !out3) return OXPT_E_ARG;
+    if (n == 0) return OXPT_OK;
+    if (n > (1u << 26)) return fail(c, OXPT_E_ARG, "debug: too many coordinates");
+    // Level sizes are 16-bit fields of the descriptors; modes 0..2.
+    for (size_t i = 0; i < n; i++)
+        if (size[i] < 1 || size[i] > 65535 || mode[i] > 2) return fail(c, OXPT_E_ARG, "debug: level size or wrap mode out of range");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    void *d = nullptr;
+    size_t bytes = n * 24;
+    if (int rc = dalloc(c, bytes, &d, "debug buffers")) return rc;
+    float *du = (float *)d;
+    int32_t *ds = (int32_t *)(du + n);
+    uint32_t *dm = (uint32_t *)(ds + n);
+    int32_t *dout = (int32_t *)(dm + n);
+    hipError_t e = hipMemcpyAsync(du, u, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, size, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dm, mode, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = oxd::launch_debug_texel_index((uint32_t)n, du, ds, dm, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out3, dout, n * 12, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(c, d, bytes);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "debug texel index", e);
+}
+
 int oxpt_debug_closest_hit(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, size_t n, const float *o3,
                            const float *d3, uint32_t *out2) {
     if (!c || !s || !P || !o3 || !d3 || !out2 || s->ctx != c) return OXPT_E_ARG;

@@ -217,6 +217,9 @@ _sig("oxpt_autotune", C.c_int, _vp, _vp, P(Params), u32, P(Tuning), P(C.c_double
 _sig("oxpt_debug_pattern_words", C.c_int, _vp, _vp, u32, sz, P(u32), P(u32), P(u32), P(u32), P(u32))
 _sig("oxpt_debug_light_picks", C.c_int, _vp, _vp, P(Params), sz, P(f32), P(u32))
 _sig("oxpt_debug_closest_hit", C.c_int, _vp, _vp, P(Params), sz, P(f32), P(f32), P(u32))
+_sig("oxpt_udiv_magic", None, u32, P(u32), P(u32))
+_sig("oxpt_debug_udiv", C.c_int, _vp, sz, P(u32), P(u32), P(u32))
+_sig("oxpt_debug_texel_index", C.c_int, _vp, sz, P(f32), P(C.c_int32), P(u32), P(C.c_int32))
 _sig("oxpt_build_wide_bvh", C.c_int, P(f32), u32, P(f32), sz, P(sz), P(u32))
 # Layer 2.
 _sig("oxpt_render_options_default", None, P(COptions))
@@ -678,6 +681,28 @@ class GpuPathTracer:
             _raise(rc, lib.oxpt_last_error(self.ctx).decode())
         return out
 
+    def debug_udiv(self, num, den) -> np.ndarray:
+        """num // den on the device with the claim step's multiply-shift division."""
+        num = np.ascontiguousarray(num, dtype=np.uint32)
+        den = np.ascontiguousarray(den, dtype=np.uint32)
+        out = np.zeros(num.size, dtype=np.uint32)
+        rc = lib.oxpt_debug_udiv(self.ctx, num.size, _ptr(num, u32), _ptr(den, u32), _ptr(out, u32))
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        return out
+
+    def debug_texel_index(self, u, size, mode) -> np.ndarray:
+        """{nearest, bilinear x0, x0 + 1} wrapped texel indices as the kernels form them."""
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        size = np.ascontiguousarray(size, dtype=np.int32)
+        mode = np.ascontiguousarray(mode, dtype=np.uint32)
+        out = np.zeros((u.size, 3), dtype=np.int32)
+        rc = lib.oxpt_debug_texel_index(self.ctx, u.size, _ptr(u, f32), _ptr(size, C.c_int32), _ptr(mode, u32),
+                                        _ptr(out, C.c_int32))
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        return out
+
     def debug_closest_hit(self, o, d) -> np.ndarray:
         o = np.ascontiguousarray(o, dtype=np.float32).reshape(-1, 3)
         d = np.ascontiguousarray(d, dtype=np.float32).reshape(-1, 3)
@@ -984,3 +1009,10 @@ def build_wide_bvh(node_words: np.ndarray) -> Tuple[np.ndarray, int]:
     if rc:
         _raise(rc, "build_wide_bvh")
     return out, depth.value
+
+
+def udiv_magic(d: int) -> Tuple[int, int]:
+    """The claim step's multiply-shift pair (m, l) for divisor d (host only)."""
+    m, l = u32(), u32()
+    lib.oxpt_udiv_magic(d, C.byref(m), C.byref(l))
+    return m.value, l.value

@@ -20,3 +20,17 @@ def gpu():
     import oxpt
 
     return oxpt.GpuPathTracer(0)
+
+
+@pytest.fixture(autouse=True)
+def _restore_gpu_state(request):
+    """The session tracer is shared: whatever a test changed (kernel form,
+    stack split, tail, path cap, environment) is put back after it, pass or
+    fail, so one failure cannot change what later tests run."""
+    yield
+    if "gpu" not in request.fixturenames:
+        return
+    g = request.getfixturevalue("gpu")
+    g.set_tuning(block=256, waves_per_cu=32, max_paths=1 << 30)
+    g.set_environment(None)
+    g.invalidate_scene()

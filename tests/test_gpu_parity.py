@@ -165,14 +165,40 @@ def test_terrain_env_psnr(gpu):
 
 
 def test_plain_kernel_matches_persistent(gpu):
-    """Both launch forms add each pixel's samples in order: identical accumulators."""
+    """Both launch forms add each pixel's samples in order: identical
+    accumulators. The persistent form is forced (kernel 3: 4 waves per SIMD),
+    since a default refine of a frame smaller than the GPU's resident lanes
+    runs as a one-job batch; the kernel that ran is asserted."""
     scene = oxpt.Scene.room(0)
     opts = oxpt.RenderOptions(width=72, height=40, samples_per_pixel=6, max_bounces=8)
     gpu.set_tuning(kernel=1)
     a = gpu.render_hdr(scene, opts)
-    gpu.set_tuning(kernel=0)
+    assert gpu.last_kernel() == "trace_tiles", gpu.last_kernel()
+    gpu.set_tuning(kernel=3)
     b = gpu.render_hdr(scene, opts)
+    assert gpu.last_kernel().startswith("trace_persistent"), gpu.last_kernel()
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    gpu.set_tuning(kernel=0)
+    c = gpu.render_hdr(scene, opts)
+    assert gpu.last_kernel().startswith("trace_batch"), gpu.last_kernel()
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
+def test_default_form_of_a_large_frame_is_persistent(gpu):
+    """A frame of at least the GPU's resident lanes (512 x 544 px > 256 CUs x
+    4 SIMDs x 4 waves x 64 lanes) takes the default persistent megakernel with
+    its path tail — the form the single-large-scene bench measures — and
+    equals the plain per-pixel kernel bit for bit."""
+    scene = small_terrain()
+    env = oxpt.EnvironmentMap.sky(128, 64, 2)
+    opts = oxpt.RenderOptions(width=512, height=544, samples_per_pixel=3, max_bounces=8)
+    gpu.set_environment(env)
+    gpu.set_tuning(kernel=1)
+    ref = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+    gpu.set_tuning(kernel=0)
+    got = gpu.render_hdr(scene, opts).view(np.uint32)
+    assert gpu.last_kernel().startswith("trace_persistent"), gpu.last_kernel()
+    assert np.array_equal(got, ref)
 
 
 def test_punctual_lights_forms_bitwise(gpu):
