@@ -65,6 +65,14 @@ typedef struct oxpt_scene_desc {
 /* Opens device `device_ordinal` (open_device_blocking, src/gpu.rs:987-1056).
  * OXPT_E_BACKEND when no such HIP device exists. */
 int oxpt_ctx_create(int device_ordinal, oxpt_ctx **out);
+/* A second context on base's device with its own stream (and its own launch
+ * scratch), sharing base's objects and memory budget: scenes and
+ * accumulators made through either may be used with the other, so uploads
+ * and readbacks on one overlap launches on the other. Work on the two
+ * streams is not ordered; the caller orders it (finish). Objects belong to
+ * base: destroy peers and objects before base. No counterpart in the
+ * reference (wgpu has one queue per device). */
+int oxpt_ctx_create_peer(oxpt_ctx *base, oxpt_ctx **out);
 void oxpt_ctx_destroy(oxpt_ctx *ctx);
 /* Text of the last failure on this context ("" if none). */
 const char *oxpt_last_error(const oxpt_ctx *ctx);
@@ -516,23 +524,37 @@ int oxpt_packed_info(const oxpt_packed *p, uint32_t *atlas_side, uint32_t *atlas
 int oxpt_build_wide_bvh(const float *node_words, uint32_t node_count, float *out, size_t out_vec4_cap,
                         size_t *out_vec4, uint32_t *wide_depth);
 
-/* Serving (continuous batching). A pool of prepared scenes; render requests
+/* Serving: a continuous-batching loop over oxpt_refine_batch. Render requests
  * name a pool scene and options, refine requests add samples to a finished
- * render's resident accumulator. Requests are admitted against an HBM
- * budget (LRU eviction of idle scenes), and every tick one batched launch
- * advances all active requests by at most spp_per_tick samples; requests
- * arriving meanwhile join the next tick. Images equal the images rendered
- * alone (per-pixel sample order is kept). No counterpart in the reference. */
+ * render's resident accumulator. Requests are admitted against an HBM budget
+ * that counts scenes, accumulators, readback staging and the launch scratch
+ * (LRU eviction of idle scenes; a request that can never fit is rejected).
+ * Scene uploads, accumulator creation and result readbacks run on a second
+ * stream from an I/O thread, overlapping the running launch; a finished
+ * request's resolve is queued right after the launch that finished it. Each
+ * tick is one batched launch over every active request: with tick_ms > 0 it
+ * is sized to about that duration from the measured rate, and the requests
+ * closest to done get their remaining samples first (every request advances
+ * at least one sample per tick); with tick_ms = 0 every request advances
+ * spp_per_tick samples. Requests arriving meanwhile join the next tick.
+ * Images equal the images rendered alone (per-pixel sample order is kept).
+ * No counterpart in the reference. */
 typedef struct oxpt_server oxpt_server;
 typedef struct oxpt_server_config {
-    uint64_t hbm_budget;    /* bytes for resident scenes + accumulators (default 200 GiB) */
-    uint32_t spp_per_tick;  /* samples per request per batched launch (default 16) */
+    uint64_t hbm_budget;    /* device bytes the server may hold (default 200 GiB) */
+    uint32_t spp_per_tick;  /* tick_ms = 0: samples per request per tick (default 16);
+                               tick_ms > 0: a cap per request per tick (0 = none) */
     uint32_t max_active;    /* request slots, 0 = unbounded */
     uint32_t keep_hdr;      /* also keep the RGBA32F result */
     uint32_t use_graph;     /* replay the batched step from a captured HIP graph */
     uint32_t device_build;  /* keep pool scenes as host scenes and prepare them on the device at first
                                use (oxpt_scene_build + device environment tables) instead of packing
                                them on the host in oxpt_server_add_scene; same images */
+    uint32_t tick_ms;       /* target tick duration in ms (0: fixed spp_per_tick; default 0) */
+    uint64_t tick_paths;    /* most paths (8x8-tile padded pixels x samples) one tick traces; bounds its
+                               per-path sample buffer at 16 B each (default 2^27, at most 2^31) */
+    uint32_t fail_tick;     /* testing: tick number n (1-based) fails without launching (0 = never) */
+    uint32_t pad;
 } oxpt_server_config;
 typedef struct oxpt_request_info {
     uint32_t ok, width, height, samples;
@@ -543,8 +565,10 @@ typedef struct oxpt_request_info {
 typedef struct oxpt_server_stats {
     uint64_t submitted, completed, rejected, ticks, uploads, evictions, evicted_accums, paths;
     double busy_s;          /* wall time inside batched launches */
-    uint64_t resident_bytes;
+    uint64_t resident_bytes;  /* scenes + accumulators + staging + reserved launch scratch */
     uint32_t active, queued;
+    uint64_t failed_ticks;
+    double io_s;            /* I/O thread time in uploads and readbacks (overlaps launches) */
 } oxpt_server_stats;
 void oxpt_server_config_default(oxpt_server_config *c);
 int oxpt_server_create(int device, const oxpt_server_config *cfg, oxpt_server **out);

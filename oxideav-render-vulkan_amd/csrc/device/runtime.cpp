@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -79,6 +80,13 @@ struct EventPair {
     hipEvent_t a, b;
 };
 
+// Device memory shared by a context and its peers: one budget, one count
+// of bytes in use (contexts on different threads allocate concurrently).
+struct Domain {
+    std::mutex mu;
+    uint64_t budget = 0, in_use = 0;
+};
+
 }  // namespace
 
 struct oxpt_ctx {
@@ -86,7 +94,10 @@ struct oxpt_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::string summary;
-    uint64_t budget = 0, in_use = 0;
+    // The context that owns the objects created through this one (itself
+    // unless this is a peer), and the memory domain they share.
+    oxpt_ctx *base = nullptr;
+    std::shared_ptr<Domain> dom;
     uint32_t n_cu = 0;
     oxpt_tuning tuning{256, 32, 1u << 30, 0, 0, 0, 0, 0, 0};
     uint32_t *d_work = nullptr;
@@ -163,18 +174,24 @@ int hip_fail(oxpt_ctx *c, const char *what, hipError_t e) {
 int dalloc(oxpt_ctx *c, size_t bytes, void **p, const char *what) {
     *p = nullptr;
     if (bytes == 0) bytes = 16;
-    if (c->in_use + bytes > c->budget) {
-        char buf[256];
-        std::snprintf(buf, sizeof buf, "hip: %s needs %zu bytes; the device budget leaves %llu", what, bytes,
-                      (unsigned long long)(c->budget > c->in_use ? c->budget - c->in_use : 0));
-        return fail(c, OXPT_E_BACKEND, buf);
+    Domain &D = *c->dom;
+    {
+        std::lock_guard<std::mutex> lk(D.mu);
+        if (D.in_use + bytes > D.budget) {
+            char buf[256];
+            std::snprintf(buf, sizeof buf, "hip: %s needs %zu bytes; the device budget leaves %llu", what, bytes,
+                          (unsigned long long)(D.budget > D.in_use ? D.budget - D.in_use : 0));
+            return fail(c, OXPT_E_BACKEND, buf);
+        }
+        D.in_use += bytes;
     }
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
+        std::lock_guard<std::mutex> lk(D.mu);
+        D.in_use -= std::min<uint64_t>(D.in_use, bytes);
         return hip_fail(c, what, e);
     }
-    c->in_use += bytes;
     return OXPT_OK;
 }
 
@@ -182,8 +199,12 @@ void dfree(oxpt_ctx *c, void *p, size_t bytes) {
     if (!p) return;
     (void)hipFree(p);
     if (bytes == 0) bytes = 16;
-    c->in_use -= std::min<uint64_t>(c->in_use, bytes);
+    std::lock_guard<std::mutex> lk(c->dom->mu);
+    c->dom->in_use -= std::min<uint64_t>(c->dom->in_use, bytes);
 }
+
+// Objects (scenes, accumulators) of one context may be used with its peers.
+inline bool same_domain(const oxpt_ctx *a, const oxpt_ctx *b) { return a && b && a->dom == b->dom; }
 
 int setdev(oxpt_ctx *c) {
     hipError_t e = hipSetDevice(c->device);
@@ -202,7 +223,7 @@ void fold_events(oxpt_ctx *c, size_t keep) {
     }
 }
 
-oxd::TraceArgs trace_args(const oxpt_scene *s, oxpt_accum *acc, const oxpt_params *P) {
+oxd::TraceArgs trace_args(const oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *acc, const oxpt_params *P) {
     oxd::TraceArgs a;
     std::memset(&a, 0, sizeof a);
     a.P = *P;
@@ -218,8 +239,8 @@ oxd::TraceArgs trace_args(const oxpt_scene *s, oxpt_accum *acc, const oxpt_param
         a.n_items = 64u * tx * ty;
         a.split = a.n_items;  // no tail unless the launch sets one
     }
-    a.work = s->ctx->d_work;
-    if (s->ctx->tuning.bvh == 2) a.P.offs2[3] = 0;  // walk the binary tree
+    a.work = c->d_work;
+    if (c->tuning.bvh == 2) a.P.offs2[3] = 0;  // walk the binary tree
     return a;
 }
 
@@ -473,7 +494,9 @@ int oxpt_ctx_create(int ordinal, oxpt_ctx **out) {
         return OXPT_E_BACKEND;
     }
     c->n_cu = (uint32_t)prop.multiProcessorCount;
-    c->budget = (uint64_t)prop.totalGlobalMem;
+    c->base = c;
+    c->dom = std::make_shared<Domain>();
+    c->dom->budget = (uint64_t)prop.totalGlobalMem;
     std::string name = prop.name;
     if (name.empty()) {
         char buf[256] = {0};
@@ -490,6 +513,34 @@ int oxpt_ctx_create(int ordinal, oxpt_ctx **out) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return OXPT_E_BACKEND;
+    }
+    c->d_work = static_cast<uint32_t *>(w);
+    *out = c;
+    return OXPT_OK;
+}
+
+int oxpt_ctx_create_peer(oxpt_ctx *base, oxpt_ctx **out) {
+    if (!base || !out) return OXPT_E_ARG;
+    *out = nullptr;
+    std::lock_guard<std::mutex> lock(g_device_init);
+    if (setdev(base)) return OXPT_E_BACKEND;
+    auto *c = new oxpt_ctx();
+    c->device = base->device;
+    c->n_cu = base->n_cu;
+    c->summary = base->summary;
+    c->tuning = base->tuning;
+    c->base = base->base;
+    c->dom = base->dom;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(base, OXPT_E_BACKEND, "hip: peer stream");
+    }
+    void *w = nullptr;
+    if (int rc = dalloc(c, 256, &w, "work counter")) {
+        (void)hipStreamDestroy(c->stream);
+        base->err = c->err;
+        delete c;
+        return rc;
     }
     c->d_work = static_cast<uint32_t *>(w);
     *out = c;
@@ -532,10 +583,15 @@ const char *oxpt_ctx_last_kernel(const oxpt_ctx *c) { return c ? c->last_kernel 
 
 int oxpt_ctx_set_budget(oxpt_ctx *c, uint64_t bytes) {
     if (!c) return OXPT_E_ARG;
-    c->budget = bytes;
+    std::lock_guard<std::mutex> lk(c->dom->mu);
+    c->dom->budget = bytes;
     return OXPT_OK;
 }
-uint64_t oxpt_ctx_bytes_in_use(const oxpt_ctx *c) { return c ? c->in_use : 0; }
+uint64_t oxpt_ctx_bytes_in_use(const oxpt_ctx *c) {
+    if (!c) return 0;
+    std::lock_guard<std::mutex> lk(c->dom->mu);
+    return c->dom->in_use;
+}
 
 int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
     if (!c || !t) return OXPT_E_ARG;
@@ -585,7 +641,7 @@ int oxpt_scene_upload(oxpt_ctx *c, const oxpt_scene_desc *d, oxpt_scene **out) {
         return fail(c, OXPT_E_ARG, "scene: geom, attrs and tables are required");
     if (setdev(c)) return OXPT_E_BACKEND;
     auto *s = new oxpt_scene();
-    s->ctx = c;
+    s->ctx = c->base;
     s->table_words = d->table_words;
     s->geom_vec4 = d->geom_vec4;
     s->attrs_vec4 = d->attrs_vec4;
@@ -744,7 +800,7 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
     }
     release();
     auto *s = new oxpt_scene();
-    s->ctx = c;
+    s->ctx = c->base;
     s->geom = bg.geom;
     s->attrs = bg.attrs;
     s->geom_vec4 = bg.geom_vec4;
@@ -801,7 +857,7 @@ int oxpt_scene_build(oxpt_ctx *c, const oxpt_soup_desc *d, uint32_t *emissive_sl
 }
 
 int oxpt_scene_read(oxpt_ctx *c, const oxpt_scene *s, int which, void *host_out, size_t bytes, size_t *words) {
-    if (!c || !s || s->ctx != c) return OXPT_E_ARG;
+    if (!c || !s || !same_domain(s->ctx, c)) return OXPT_E_ARG;
     const void *src;
     size_t count, elem;
     switch (which) {
@@ -821,7 +877,7 @@ int oxpt_scene_read(oxpt_ctx *c, const oxpt_scene *s, int which, void *host_out,
 }
 
 int oxpt_scene_set_tables(oxpt_ctx *c, oxpt_scene *s, const uint32_t *tables, size_t words) {
-    if (!c || !s || !tables || words == 0 || s->ctx != c) return OXPT_E_ARG;
+    if (!c || !s || !tables || words == 0 || !same_domain(s->ctx, c)) return OXPT_E_ARG;
     if (setdev(c)) return OXPT_E_BACKEND;
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, "tables update", e);
@@ -841,7 +897,7 @@ int oxpt_scene_set_tables(oxpt_ctx *c, oxpt_scene *s, const uint32_t *tables, si
 
 int oxpt_scene_set_tables_env(oxpt_ctx *c, oxpt_scene *s, const uint32_t *tables, size_t words,
                               const oxpt_env_desc *env) {
-    if (!c || !s || !tables || words == 0 || !env || s->ctx != c) return OXPT_E_ARG;
+    if (!c || !s || !tables || words == 0 || !env || !same_domain(s->ctx, c)) return OXPT_E_ARG;
     if (!env->rgb || !env->row_weight || env->w == 0 || env->h == 0 || (uint64_t)env->w * env->h > (1ull << 28))
         return fail(c, OXPT_E_ARG, "environment: rgb, row weights and 1 .. 2^28 texels are required");
     const size_t w = env->w, h = env->h, n = w * h;
@@ -890,12 +946,14 @@ void oxpt_scene_destroy(oxpt_scene *s) {
     oxpt_ctx *c = s->ctx;
     setdev(c);
     (void)hipStreamSynchronize(c->stream);
-    uint64_t before = c->in_use;
     (void)hipFree(s->geom);
     (void)hipFree(s->attrs);
     (void)hipFree(s->tables);
     (void)hipFree(s->atlas);
-    c->in_use = before - std::min<uint64_t>(before, s->bytes);
+    {
+        std::lock_guard<std::mutex> lk(c->dom->mu);
+        c->dom->in_use -= std::min<uint64_t>(c->dom->in_use, s->bytes);
+    }
     delete s;
 }
 
@@ -905,7 +963,7 @@ int oxpt_accum_create(oxpt_ctx *c, uint32_t w, uint32_t h, oxpt_accum **out) {
     if (w == 0 || h == 0) return fail(c, OXPT_E_INVALID, "accumulator: zero dimension");
     if (setdev(c)) return OXPT_E_BACKEND;
     auto *a = new oxpt_accum();
-    a->ctx = c;
+    a->ctx = c->base;
     a->w = w;
     a->h = h;
     a->bytes = (size_t)w * h * 16;
@@ -935,14 +993,14 @@ void oxpt_accum_destroy(oxpt_accum *a) {
 }
 
 int oxpt_accum_clear(oxpt_ctx *c, oxpt_accum *a) {
-    if (!c || !a || a->ctx != c) return OXPT_E_ARG;
+    if (!c || !a || !same_domain(a->ctx, c)) return OXPT_E_ARG;
     if (setdev(c)) return OXPT_E_BACKEND;
     hipError_t e = hipMemsetAsync(a->d, 0, a->bytes, c->stream);
     return e == hipSuccess ? OXPT_OK : hip_fail(c, "accumulator clear", e);
 }
 
 int oxpt_accum_read(oxpt_ctx *c, const oxpt_accum *a, float *host, size_t bytes) {
-    if (!c || !a || !host || a->ctx != c) return OXPT_E_ARG;
+    if (!c || !a || !host || !same_domain(a->ctx, c)) return OXPT_E_ARG;
     if (bytes < a->bytes) return fail(c, OXPT_E_ARG, "accumulator read: buffer too small");
     if (setdev(c)) return OXPT_E_BACKEND;
     hipError_t e = hipMemcpyAsync(host, a->d, a->bytes, hipMemcpyDeviceToHost, c->stream);
@@ -951,7 +1009,7 @@ int oxpt_accum_read(oxpt_ctx *c, const oxpt_accum *a, float *host, size_t bytes)
 }
 
 int oxpt_accum_write(oxpt_ctx *c, oxpt_accum *a, const float *host, size_t bytes) {
-    if (!c || !a || !host || a->ctx != c) return OXPT_E_ARG;
+    if (!c || !a || !host || !same_domain(a->ctx, c)) return OXPT_E_ARG;
     if (bytes < a->bytes) return fail(c, OXPT_E_ARG, "accumulator write: buffer too small");
     if (setdev(c)) return OXPT_E_BACKEND;
     hipError_t e = hipMemcpyAsync(a->d, host, a->bytes, hipMemcpyHostToDevice, c->stream);
@@ -1000,7 +1058,7 @@ static int ensure_samples(oxpt_ctx *c, uint64_t slots) {
 }
 
 int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_params *P, uint32_t first, uint32_t count) {
-    if (!c || !s || !a || !P || s->ctx != c || a->ctx != c) return OXPT_E_ARG;
+    if (!c || !s || !a || !P || !same_domain(s->ctx, c) || !same_domain(a->ctx, c)) return OXPT_E_ARG;
     if (P->dims[0] != a->w || P->dims[1] != a->h) return fail(c, OXPT_E_ARG, "refine: params size differs from the accumulator");
     if (count == 0) return OXPT_OK;
     if (int rc = check_params(c, s, P)) return rc;
@@ -1070,7 +1128,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     uint64_t end = (uint64_t)first + count;
     for (uint64_t f = first; f < end;) {
         uint32_t n = (uint32_t)std::min<uint64_t>(per, end - f);
-        oxd::TraceArgs args = trace_args(s, a, P);
+        oxd::TraceArgs args = trace_args(c, s, a, P);
         args.first = (uint32_t)f;
         args.count = n;
         if (persistent && n > 1 && tail16 > 0) {
@@ -1114,7 +1172,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
     const oxpt_batch_item &i0 = items[0];
     for (size_t i = 0; i < n; i++) {
         const oxpt_batch_item &it = items[i];
-        if (!it.scene || !it.accum || it.scene->ctx != c || it.accum->ctx != c) return fail(c, OXPT_E_ARG, "batch: item belongs to another context");
+        if (!it.scene || !it.accum || !same_domain(it.scene->ctx, c) || !same_domain(it.accum->ctx, c)) return fail(c, OXPT_E_ARG, "batch: item belongs to another context");
         if (it.params.dims[0] != it.accum->w || it.params.dims[1] != it.accum->h)
             return fail(c, OXPT_E_ARG, "batch: params size differs from the accumulator");
         if (int rc = check_params(c, it.scene, &it.params)) return rc;
@@ -1325,7 +1383,7 @@ int oxpt_finish(oxpt_ctx *c) {
 
 int oxpt_autotune(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, uint32_t spp_per_trial,
                   oxpt_tuning *best_out, double *best_ms) {
-    if (!c || !s || !P || s->ctx != c) return OXPT_E_ARG;
+    if (!c || !s || !P || !same_domain(s->ctx, c)) return OXPT_E_ARG;
     if (int rc = check_params(c, s, P)) return rc;
     if (setdev(c)) return OXPT_E_BACKEND;
     const uint32_t spp = std::max(1u, std::min(spp_per_trial, 64u));
@@ -1403,7 +1461,7 @@ int oxpt_ctx_timing(oxpt_ctx *c, int reset, uint64_t *launches, double *total_ms
 }
 
 int oxpt_resolve(oxpt_ctx *c, const oxpt_accum *a, const oxpt_resolve_params *rp, void *host, size_t bytes) {
-    if (!c || !a || !rp || !host || a->ctx != c) return OXPT_E_ARG;
+    if (!c || !a || !rp || !host || !same_domain(a->ctx, c)) return OXPT_E_ARG;
     size_t need = (size_t)a->w * a->h * (rp->cfg[3] == 1 ? 16 : 4);
     if (bytes < need) return fail(c, OXPT_E_ARG, "resolve: output buffer too small");
     if (setdev(c)) return OXPT_E_BACKEND;
@@ -1421,7 +1479,7 @@ int oxpt_resolve(oxpt_ctx *c, const oxpt_accum *a, const oxpt_resolve_params *rp
 }
 
 int oxpt_resolve_device(oxpt_ctx *c, const oxpt_accum *a, const oxpt_resolve_params *rp, void *dev, size_t bytes) {
-    if (!c || !a || !rp || !dev || a->ctx != c) return OXPT_E_ARG;
+    if (!c || !a || !rp || !dev || !same_domain(a->ctx, c)) return OXPT_E_ARG;
     const size_t need = (size_t)a->w * a->h * (rp->cfg[3] == 1 ? 16 : 4);
     if (bytes < need) return fail(c, OXPT_E_ARG, "resolve: output buffer too small");
     if (setdev(c)) return OXPT_E_BACKEND;
@@ -1441,9 +1499,59 @@ int oxpt_resolve_device(oxpt_ctx *c, const oxpt_accum *a, const oxpt_resolve_par
     return e == hipSuccess ? OXPT_OK : hip_fail(c, "resolve", e);
 }
 
+}  // extern "C"
+
+// Stream-ordered helpers of the serving loop (declared in scene.hpp).
+namespace oxpt {
+
+int ctx_mark(oxpt_ctx *c, void **mark) {
+    *mark = nullptr;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipEvent_t ev = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(ev, c->stream);
+    if (e != hipSuccess) {
+        if (ev) (void)hipEventDestroy(ev);
+        return hip_fail(c, "mark", e);
+    }
+    *mark = ev;
+    return OXPT_OK;
+}
+
+int mark_wait(oxpt_ctx *c, void *mark) {
+    if (!mark) return OXPT_OK;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipEvent_t ev = static_cast<hipEvent_t>(mark);
+    hipError_t e = hipEventSynchronize(ev);
+    (void)hipEventDestroy(ev);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "mark wait", e);
+}
+
+int device_alloc(oxpt_ctx *c, size_t bytes, void **p) {
+    if (setdev(c)) return OXPT_E_BACKEND;
+    return dalloc(c, bytes, p, "staging buffer");
+}
+
+void device_free(oxpt_ctx *c, void *p, size_t bytes) {
+    if (!p) return;
+    setdev(c);
+    dfree(c, p, bytes);
+}
+
+int copy_to_host(oxpt_ctx *c, void *host, const void *dev, size_t bytes) {
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipError_t e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? OXPT_OK : hip_fail(c, "readback", e);
+}
+
+}  // namespace oxpt
+
+extern "C" {
+
 int oxpt_debug_pattern_words(oxpt_ctx *c, const oxpt_scene *s, uint32_t seed, size_t n, const uint32_t *x,
                              const uint32_t *y, const uint32_t *index, const uint32_t *p, uint32_t *out4) {
-    if (!c || !s || !x || !y || !index || !p || !out4 || s->ctx != c) return OXPT_E_ARG;
+    if (!c || !s || !x || !y || !index || !p || !out4 || !same_domain(s->ctx, c)) return OXPT_E_ARG;
     if (n == 0) return OXPT_OK;
     if (n > (1u << 28)) return fail(c, OXPT_E_ARG, "debug: too many tuples");
     if (setdev(c)) return OXPT_E_BACKEND;
@@ -1466,7 +1574,7 @@ int oxpt_debug_pattern_words(oxpt_ctx *c, const oxpt_scene *s, uint32_t seed, si
 
 int oxpt_debug_light_picks(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, size_t n, const float *u,
                            uint32_t *out3) {
-    if (!c || !s || !P || !u || !out3 || s->ctx != c) return OXPT_E_ARG;
+    if (!c || !s || !P || !u || !out3 || !same_domain(s->ctx, c)) return OXPT_E_ARG;
     if (n == 0) return OXPT_OK;
     if (n > (1u << 28)) return fail(c, OXPT_E_ARG, "debug: too many uniforms");
     if (int rc = check_params(c, s, P)) return rc;
@@ -1476,7 +1584,7 @@ int oxpt_debug_light_picks(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *
     if (int rc = dalloc(c, bytes, &d, "debug buffers")) return rc;
     float *du = (float *)d;
     uint32_t *dout = (uint32_t *)(du + n);
-    oxd::TraceArgs args = trace_args(s, nullptr, P);
+    oxd::TraceArgs args = trace_args(c, s, nullptr, P);
     hipError_t e = hipMemcpyAsync(du, u, n * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = oxd::launch_debug_picks(args, (uint32_t)n, du, dout, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out3, dout, n * 12, hipMemcpyDeviceToHost, c->stream);
@@ -1542,7 +1650,7 @@ int oxpt_debug_texel_index(oxpt_ctx *c, size_t n, const float *u, const int32_t 
 
 int oxpt_debug_closest_hit(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, size_t n, const float *o3,
                            const float *d3, uint32_t *out2) {
-    if (!c || !s || !P || !o3 || !d3 || !out2 || s->ctx != c) return OXPT_E_ARG;
+    if (!c || !s || !P || !o3 || !d3 || !out2 || !same_domain(s->ctx, c)) return OXPT_E_ARG;
     if (n == 0) return OXPT_OK;
     if (n > (1u << 26)) return fail(c, OXPT_E_ARG, "debug: too many rays");
     if (setdev(c)) return OXPT_E_BACKEND;
@@ -1551,7 +1659,7 @@ int oxpt_debug_closest_hit(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *
     if (int rc = dalloc(c, bytes, &d, "debug buffers")) return rc;
     float *dorig = (float *)d, *ddir = dorig + 3 * n;
     uint32_t *dout = (uint32_t *)(ddir + 3 * n);
-    oxd::TraceArgs args = trace_args(s, nullptr, P);
+    oxd::TraceArgs args = trace_args(c, s, nullptr, P);
     hipError_t e = hipMemcpyAsync(dorig, o3, n * 12, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(ddir, d3, n * 12, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = oxd::launch_debug_hits(args, (uint32_t)n, dorig, ddir, dout, c->stream);

@@ -141,6 +141,17 @@ int device_prepare(oxpt_ctx *ctx, const HostScene &s, Packed &pk, oxpt_scene **o
 // device after them when env is a full map (else the tables alone).
 int upload_tables(oxpt_ctx *ctx, oxpt_scene *dev, const std::vector<uint32_t> &scene_tables, const EnvMap *env);
 
+// Stream-ordered helpers of the serving loop (csrc/device/runtime.cpp).
+// ctx_mark records a point in ctx's stream; mark_wait blocks the calling
+// thread until the stream has passed it, then releases the mark.
+int ctx_mark(oxpt_ctx *c, void **mark);
+int mark_wait(oxpt_ctx *c, void *mark);
+// Device memory counted in the context's budget (staging for readbacks).
+int device_alloc(oxpt_ctx *c, size_t bytes, void **p);
+void device_free(oxpt_ctx *c, void *p, size_t bytes);
+// Device -> host copy on ctx's stream; returns when it is done.
+int copy_to_host(oxpt_ctx *c, void *host, const void *dev, size_t bytes);
+
 void sobol_directions(uint32_t out[OXPT_SOBOL_WORDS]);
 void sheen_table(float out[OXPT_SHEEN_WORDS]);
 uint16_t f32_to_f16(float v);

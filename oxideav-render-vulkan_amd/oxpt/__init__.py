@@ -158,7 +158,8 @@ class BatchItem(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", u64), ("spp_per_tick", u32), ("max_active", u32), ("keep_hdr", u32),
-                ("use_graph", u32), ("device_build", u32)]
+                ("use_graph", u32), ("device_build", u32), ("tick_ms", u32), ("tick_paths", u64),
+                ("fail_tick", u32), ("pad", u32)]
 
 
 class RequestInfo(C.Structure):
@@ -169,7 +170,8 @@ class RequestInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("rejected", u64), ("ticks", u64), ("uploads", u64),
                 ("evictions", u64), ("evicted_accums", u64), ("paths", u64), ("busy_s", C.c_double),
-                ("resident_bytes", u64), ("active", u32), ("queued", u32)]
+                ("resident_bytes", u64), ("active", u32), ("queued", u32), ("failed_ticks", u64),
+                ("io_s", C.c_double)]
 
 
 ABI_STRUCTS = [Params, Frame, ResolveParams, SceneDesc, TexBinding, Material, Sampler, Light, Camera,
@@ -187,6 +189,7 @@ def _sig(name, res, *args):
 
 # Layer 1.
 _sig("oxpt_ctx_create", C.c_int, C.c_int, P(_vp))
+_sig("oxpt_ctx_create_peer", C.c_int, _vp, P(_vp))
 _sig("oxpt_ctx_destroy", None, _vp)
 _sig("oxpt_last_error", C.c_char_p, _vp)
 _sig("oxpt_ctx_summary", C.c_char_p, _vp)
@@ -877,12 +880,15 @@ class Server:
     """Continuous-batching render server over one device (include/oxpt/oxpt.h, serving)."""
 
     def __init__(self, device=0, hbm_budget=200 << 30, spp_per_tick=16, max_active=0, keep_hdr=False,
-                 use_graph=False, device_build=False):
+                 use_graph=False, device_build=False, tick_ms=0, tick_paths=0, fail_tick=0):
         cfg = ServerConfig()
         lib.oxpt_server_config_default(C.byref(cfg))
         cfg.hbm_budget, cfg.spp_per_tick, cfg.max_active = hbm_budget, spp_per_tick, max_active
         cfg.keep_hdr, cfg.use_graph = 1 if keep_hdr else 0, 1 if use_graph else 0
         cfg.device_build = 1 if device_build else 0
+        cfg.tick_ms, cfg.fail_tick = tick_ms, fail_tick
+        if tick_paths:
+            cfg.tick_paths = tick_paths
         h = _vp()
         rc = lib.oxpt_server_create(device, C.byref(cfg), C.byref(h))
         if rc:
