@@ -251,6 +251,19 @@ typedef struct oxpt_batch_item {
     uint32_t first, count;
 } oxpt_batch_item;
 int oxpt_refine_batch(oxpt_ctx *ctx, const oxpt_batch_item *items, size_t n);
+/* oxpt_refine_batch that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * accumulators as oxpt_refine_batch. Its job table, work counters and
+ * per-path sample buffer live in one fixed device array of 64 MiB + 68 KiB
+ * that each GPU holds from library load to process end; it takes at most 256
+ * items, and one sample of every item (8x8-tile padded pixels) at most
+ * 4,194,304 paths (larger requests fail with OXPT_E_ARG; samples are split
+ * into rounds that fit). Rounds with punctual lights run the general kernel
+ * form (the NEE queue form would need memory). Every call on a GPU shares that
+ * array: calls that overlap on one GPU (other contexts, other threads) race.
+ * Blocks until its launches are done. */
+int oxpt_refine_batch_noalloc(oxpt_ctx *ctx, const oxpt_batch_item *items, size_t n);
 /* Replay the batched step from a HIP graph captured once per launch shape
  * (the job table and count are written to device memory before each
  * replay). Off by default. */

@@ -89,6 +89,23 @@ struct Domain {
 
 }  // namespace
 
+// Scratch of oxpt_refine_batch_noalloc: work counters (4 KiB), the job
+// table (256 jobs) and the per-path sample buffer of a round (2^22 paths x
+// 16 B), in one device array that every GPU holds from module load to
+// process end. Host code never uses the array's name as an address: each
+// device's copy is found once with hipGetSymbolAddress (noalloc_scratch).
+namespace {
+constexpr uint32_t kNaJobs = 256;
+constexpr uint64_t kNaSlots = 1ull << 22;
+constexpr size_t kNaCounterBytes = 4096;
+constexpr size_t kNaJobBytes = (size_t)kNaJobs * 256;
+constexpr size_t kNaBytes = kNaCounterBytes + kNaJobBytes + kNaSlots * 16;
+constexpr int kNaMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNaMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
 struct oxpt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -515,6 +532,16 @@ int oxpt_ctx_create(int ordinal, oxpt_ctx **out) {
         return OXPT_E_BACKEND;
     }
     c->d_work = static_cast<uint32_t *>(w);
+    // This device's copy of the no-allocation scratch (once per device,
+    // with the device current).
+    if (ordinal < kNaMaxDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[ordinal]) {
+            void *ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[ordinal] = ptr;
+            else (void)hipGetLastError();
+        }
+    }
     *out = c;
     return OXPT_OK;
 }
@@ -1370,6 +1397,124 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         // The host staging of the next round's table must not race this copy.
         e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch", e);
+    }
+    return OXPT_OK;
+}
+
+int oxpt_refine_batch_noalloc(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
+    if (!c || (!items && n)) return OXPT_E_ARG;
+    if (n == 0) return OXPT_OK;
+    if (n > kNaJobs) return fail(c, OXPT_E_ARG, "batch (no allocation): more than 256 items");
+    const oxpt_batch_item &i0 = items[0];
+    uint64_t pix64 = 0;
+    uint32_t max_count = 0;
+    for (size_t i = 0; i < n; i++) {
+        const oxpt_batch_item &it = items[i];
+        if (!it.scene || !it.accum || !same_domain(it.scene->ctx, c) || !same_domain(it.accum->ctx, c))
+            return fail(c, OXPT_E_ARG, "batch: item belongs to another context");
+        if (it.params.dims[0] != it.accum->w || it.params.dims[1] != it.accum->h)
+            return fail(c, OXPT_E_ARG, "batch: params size differs from the accumulator");
+        if (int rc = check_params(c, it.scene, &it.params)) return rc;
+        for (size_t k = 0; k < i; k++)
+            if (items[k].accum == it.accum) return fail(c, OXPT_E_ARG, "batch: two items share an accumulator");
+        const uint32_t *a = it.scene->host_tables.data(), *b = i0.scene->host_tables.data();
+        if (std::memcmp(a + it.params.offs0[0], b + i0.params.offs0[0], OXPT_SOBOL_WORDS * 4) != 0 ||
+            std::memcmp(a + it.params.offs0[1], b + i0.params.offs0[1], OXPT_SHEEN_WORDS * 4) != 0)
+            return fail(c, OXPT_E_ARG, "batch: jobs carry different Sobol' / sheen tables");
+        if (it.count == 0) continue;
+        pix64 += 64ull * ((it.accum->w + 7u) / 8u) * ((it.accum->h + 7u) / 8u);
+        max_count = std::max(max_count, it.count);
+    }
+    if (max_count == 0) return OXPT_OK;
+    if (pix64 > kNaSlots)
+        return fail(c, OXPT_E_ARG, "batch (no allocation): one sample of every item is more than the 4194304 paths "
+                                   "(8x8-tile padded pixels) its scratch holds");
+    if (c->tuning.kernel == 7) return fail(c, OXPT_E_ARG, "batch (no allocation): the wavefront engine allocates");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    uint8_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (c->device < kNaMaxDevices) base = static_cast<uint8_t *>(g_scratch_ptr[c->device]);
+    }
+    if (!base) return fail(c, OXPT_E_BACKEND, "batch (no allocation): no scratch address for this device");
+    uint32_t *counters = reinterpret_cast<uint32_t *>(base);
+    oxd::Job *djobs = reinterpret_cast<oxd::Job *>(base + kNaCounterBytes);
+    float4 *samples = reinterpret_cast<float4 *>(base + kNaCounterBytes + kNaJobBytes);
+    // Rounds of spd samples per item, each round's paths within the scratch
+    // (and within max_paths).
+    const uint64_t cap = std::min<uint64_t>(kNaSlots, std::max<uint64_t>(c->tuning.max_paths, 1));
+    const uint32_t spd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_count, cap / pix64));
+    const uint32_t block = c->tuning.block;
+    const uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
+    // Host staging on the stack (about 67 KiB): no heap allocation either.
+    oxd::Job jobs[kNaJobs];
+    double work[kNaJobs];
+    uint32_t order[kNaJobs];
+    for (uint32_t done = 0; done < max_count; done += spd) {
+        // Live items of this round, largest first (a stable insertion sort:
+        // std::stable_sort may allocate).
+        uint32_t nj = 0;
+        double round_paths = 0.0;
+        for (size_t i = 0; i < n; i++) {
+            const oxpt_batch_item &it = items[i];
+            if (it.count <= done) continue;
+            const double wk = (double)it.accum->w * it.accum->h * std::min(spd, it.count - done);
+            uint32_t k = nj++;
+            while (k > 0 && work[k - 1] < wk) {
+                work[k] = work[k - 1];
+                order[k] = order[k - 1];
+                k--;
+            }
+            work[k] = wk;
+            order[k] = (uint32_t)i;
+            round_paths += wk;
+        }
+        uint32_t max_punct = 0, sample_base = 0;
+        for (uint32_t k = 0; k < nj; k++) {
+            const oxpt_batch_item &it = items[order[k]];
+            oxd::Job &J = jobs[k];
+            std::memset(&J, 0, sizeof J);
+            J.P = it.params;
+            if (c->tuning.bvh == 2) J.P.offs2[3] = 0;
+            J.geom = it.scene->geom;
+            J.attrs = it.scene->attrs;
+            J.tables = it.scene->tables;
+            J.atlas = it.scene->atlas;
+            J.accum = it.accum->d;
+            J.atlas_side = it.scene->atlas_side;
+            J.first = it.first + done;
+            J.count = std::min(spd, it.count - done);
+            const uint32_t tx = (it.accum->w + 7u) / 8u, ty = (it.accum->h + 7u) / 8u;
+            J.tiles_x = tx;
+            J.n_items = 64u * tx * ty;
+            const oxd::UDiv dt = oxd::udiv_magic(64u * J.count), dx = oxd::udiv_magic(tx);
+            J.tile_m = dt.m;
+            J.tile_l = dt.l;
+            J.tx_m = dx.m;
+            J.tx_l = dx.l;
+            J.sample_base = sample_base;
+            sample_base += J.n_items * J.count;
+            max_punct = std::max(max_punct, J.P.cfg[2]);
+        }
+        // The NEE queue of the early form with punctual lights would need
+        // memory: those rounds run the general form (the same images).
+        uint32_t mode = trace_mode(c->tuning, max_punct);
+        if (mode == 2) mode = 0;
+        oxpt_tuning form = c->tuning;
+        if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = 2;
+        else if (form.kernel == 0 && mode == 1) form.kernel = kEarlyDefaultForm;
+        const uint32_t lds = stack_lds_entries(form, block, true);
+        hipError_t e = hipMemcpyAsync(djobs, jobs, nj * sizeof(oxd::Job), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(counters + kNaJobs, &nj, 4, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return hip_fail(c, "batch (no allocation) job upload", e);
+        oxd::BatchArgs args{djobs, counters + kNaJobs, counters, kNaJobs, samples, nullptr};
+        c->last_kernel = mode == 1 ? "trace_batch(early)" : "trace_batch";
+        e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, c->stream);
+        if (e != hipSuccess) return hip_fail(c, "batch (no allocation) launch", e);
+        // The stack staging of the next round must not race this copy.
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return hip_fail(c, "batch (no allocation)", e);
     }
     return OXPT_OK;
 }
