@@ -92,6 +92,48 @@ def pmc_traffic(args, paths_per_launch):
     return int(round(per_path * paths_per_launch))
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: every CPU this process may run on,
+    at most 16 (the GPU box's CPU share per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def pmc_measured(args):
+    """Counter readings of the default kernel on this workload, from the
+    committed rocprofv3 PMC summary (None when absent or for other forms):
+    HBM bytes per path, VALU busy and lane utilisation, L2 hit rate, vector
+    store bytes per path, and the limiter they show."""
+    if (args.workload not in PMC_FILES or args.kernel != 0 or args.spp or args.block or args.waves_per_cu
+            or args.tail or args.early_off):
+        return None
+    name, kernel = PMC_FILES[args.workload]
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("kernel") != kernel:
+        return None
+    out = dict(d.get("derived", {}))
+    out["source"] = "profiles/" + name
+    return out
+
+
 def tiles(w, h, n, size=64, seed=1234):
     rng = np.random.default_rng(seed)
     out = []
@@ -103,7 +145,11 @@ def tiles(w, h, n, size=64, seed=1234):
 
 
 def bench_uneven_batch(args):
-    """64 scenes of uneven size: per-request plain launches vs one batched launch."""
+    """64 scenes of uneven size: per-request launches (the plain per-pixel
+    kernel; the regenerating persistent megakernel, forced for every frame)
+    vs one batched refine under the batch autotuner's choice, eager and
+    graph-replayed; per-job latency, PSNR against the oracle on tiles of the
+    smallest / median / largest jobs, and the oracle's own rate."""
     import oxpt
     from oxpt import workloads
 
@@ -118,53 +164,117 @@ def bench_uneven_batch(args):
     total_paths = sum(j.paths for j in jobs)
     items = [(s, a, s.params, 0, j.opts.samples_per_pixel) for j, s, a in zip(jobs, ds, acc)]
 
-    def plain():
-        dev.set_tuning(kernel=1)
-        for j, s, a in zip(jobs, ds, acc):
-            dev.refine(s, a, s.params, 0, j.opts.samples_per_pixel)
-        dev.finish()
+    def per_request(kernel):
+        def run():
+            dev.set_tuning(kernel=kernel, max_paths=1 << 30)
+            done = []
+            t0 = time.perf_counter()
+            for j, s, a in zip(jobs, ds, acc):
+                dev.refine(s, a, s.params, 0, j.opts.samples_per_pixel)
+                dev.finish()
+                done.append(time.perf_counter() - t0)
+            return done
+        return run
 
-    def per_request():
-        dev.set_tuning(kernel=0)
-        for j, s, a in zip(jobs, ds, acc):
-            dev.refine(s, a, s.params, 0, j.opts.samples_per_pixel)
-        dev.finish()
+    tuned = None
+    if args.kernel == 0:
+        dev.set_tuning(kernel=0, max_paths=1 << 30)
+        tuned, tune_ms = dev.autotune_batch(items, spp_per_trial=16)
+        log(f"autotune_batch: kernel {tuned.kernel}, max_paths {tuned.max_paths}, stack_lds {tuned.stack_lds} "
+            f"({tune_ms:.1f} ms per 16-spp trial)")
 
-    def batched():
-        dev.set_tuning(kernel=args.kernel if args.kernel != 1 else 0, block=args.block,
-                       waves_per_cu=args.waves_per_cu)
-        dev.refine_batch(items)
-        dev.finish()
+    def batched(graph):
+        def run():
+            if tuned is not None:
+                dev.set_tuning(kernel=tuned.kernel, max_paths=tuned.max_paths, stack_lds=tuned.stack_lds)
+            else:
+                dev.set_tuning(kernel=args.kernel, max_paths=1 << 30)
+            dev.set_graph(graph)
+            t0 = time.perf_counter()
+            dev.refine_batch(items)
+            dev.finish()
+            dev.set_graph(False)
+            return [time.perf_counter() - t0] * len(jobs)
+        return run
 
     res = {}
-    for name, fn in (("plain", plain), ("per_request", per_request), ("batched", batched)):
+    for name, fn in (("plain", per_request(1)), ("per_request_persistent", per_request(4)),
+                     ("batched", batched(False)), ("batched_graph", batched(True))):
         for _ in range(args.warmup):
             for a in acc:
                 a.clear()
             fn()
-        times = []
+        times, lat = [], []
         for i in range(args.steps):
             for a in acc:
                 a.clear()
             dev.finish()
             t0 = time.perf_counter()
-            fn()
+            done = fn()
             times.append(time.perf_counter() - t0)
+            lat.extend(done)
             log(f"{name} {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
-        res[name] = (sorted(times)[len(times) // 2], [a.read() for a in acc] if name == "plain" else None)
-    same = all(np_equal(a.read(), b) for a, b in zip(acc, res["plain"][1]))  # batched result vs plain
-    tp, tb, tr = res["plain"][0], res["batched"][0], res["per_request"][0]
+        res[name] = {"s": sorted(times)[len(times) // 2], "lat": lat, "kernel": dev.last_kernel(),
+                     "images": [a.read() for a in acc] if name in ("plain", "batched") else None}
+    same = all(np_equal(a, b) for a, b in zip(res["batched"]["images"], res["plain"]["images"]))
+    same_graph = all(np_equal(a.read(), b) for a, b in zip(acc, res["plain"]["images"]))
+
+    def lat_ms(name):
+        xs = sorted(x * 1e3 for x in res[name]["lat"])
+        return {"p50": round(xs[len(xs) // 2], 3), "p99": round(xs[min(len(xs) - 1, int(0.99 * (len(xs) - 1) + 0.5))], 3)}
+
+    # Oracle: PSNR on two 64x64 tiles of the smallest, median and largest
+    # jobs at full spp, and its rate on those tiles.
+    psnr_by_job, cpu_paths, cpu_s = {}, 0, 0.0
+    threads = cpu_threads()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        order = sorted(range(len(jobs)), key=lambda i: jobs[i].paths)
+        for name, i in (("smallest", order[0]), ("median", order[len(order) // 2]), ("largest", order[-1])):
+            j = jobs[i]
+            w, h, spp = j.opts.width, j.opts.height, j.opts.samples_per_pixel
+            osc = oracle.Scene.from_packed(j.packed)
+            tl = tiles(w, h, 2, seed=100 + i)
+            cacc = np.zeros((h, w, 4), np.float32)
+            tc = time.perf_counter()
+            for r in tl:
+                osc.render(0, spp, region=r, accum=cacc, threads=threads)
+            cpu_s += time.perf_counter() - tc
+            cpu_paths += sum((r[2] - r[0]) * (r[3] - r[1]) for r in tl) * spp
+            rp = oxpt.ResolveParams((1, 3, 0, 0), (1.0, float(spp), float(w), 0.0),
+                                    tuple(c / 255.0 for c in j.opts.background))
+            gi, ci = oracle.resolve(res["batched"]["images"][i], rp), oracle.resolve(cacc, rp)
+            gt = np.concatenate([gi[r[1]:r[3], r[0]:r[2]].reshape(-1, 4) for r in tl])
+            ct = np.concatenate([ci[r[1]:r[3], r[0]:r[2]].reshape(-1, 4) for r in tl])
+            psnr_by_job[f"{name} (job {i}: {j.n_tris} tris, {w}x{h}, {spp} spp)"] = round(psnr(gt, ct), 2)
+            log(f"oracle {name} job {i}: PSNR {psnr_by_job[list(psnr_by_job)[-1]]} dB")
+    tp, tr = res["plain"]["s"], res["per_request_persistent"]["s"]
+    tb, tg = res["batched"]["s"], res["batched_graph"]["s"]
+    best = min(tb, tg)
     out = {
         "metric": "Mpaths/s on the uneven batch (64 scenes): one batched refine vs per-request launches",
-        "value": round(total_paths / tb / 1e6, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(tb * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "value": round(total_paths / best / 1e6, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(best * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene generators)",
         "config": {"workload": "uneven batch: 64 terrain scenes, tris log-uniform 1k..2M, side U(128..1024), "
-                               "spp in {4,16,64,256}, depth 12, per-scene seeds", "paths": total_paths},
-        "plain_mpaths_s": round(total_paths / tp / 1e6, 3), "speedup_vs_plain": round(tp / tb, 3),
+                               "spp in {4,16,64,256}, depth 12, per-scene seeds", "paths": total_paths,
+                   "batch_tuning": None if tuned is None else {"kernel": tuned.kernel, "max_paths": tuned.max_paths,
+                                                               "stack_lds": tuned.stack_lds},
+                   "batch_kernel": res["batched"]["kernel"]},
+        "batched_mpaths_s": {"eager": round(total_paths / tb / 1e6, 3), "graph": round(total_paths / tg / 1e6, 3)},
         "per_request_persistent_mpaths_s": round(total_paths / tr / 1e6, 3),
-        "speedup_vs_per_request_persistent": round(tr / tb, 3),
-        "bitwise_equal_to_plain": bool(same),
+        "per_request_persistent_kernel": res["per_request_persistent"]["kernel"],
+        "speedup_vs_per_request_persistent": round(tr / best, 3),
+        "plain_mpaths_s": round(total_paths / tp / 1e6, 3), "speedup_vs_plain": round(tp / best, 3),
+        "latency_ms": {"batched": lat_ms("batched"), "per_request_persistent": lat_ms("per_request_persistent")},
+        "bitwise_equal_to_plain": bool(same and same_graph),
+        "psnr_db_vs_cpu": psnr_by_job or None,
+        "cpu_baseline": None if args.no_cpu else {
+            "value": round(cpu_paths / cpu_s / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"2 seeded 64x64 tiles of the smallest, median and largest jobs at full spp ({cpu_paths} "
+                      "paths), CPU oracle, std::thread row tiles", "cpu_model": cpu_model()},
     }
     print(json.dumps(out), flush=True)
 
@@ -183,7 +293,8 @@ def bench_serving(args):
     t = time.perf_counter()
     pool = workloads.uneven_batch(n=64, seed=3, threads=min(16, os.cpu_count() or 1), pack=False)
     t_gen = time.perf_counter() - t
-    srv = oxpt.Server(0, hbm_budget=args.budget_gb << 30, spp_per_tick=args.spp_tick, device_build=args.device_build)
+    srv = oxpt.Server(0, hbm_budget=args.budget_gb << 30, spp_per_tick=args.spp_tick, device_build=args.device_build,
+                      tick_ms=args.tick_ms)
     t = time.perf_counter()
     ids = [srv.add_scene(j.scene, j.env) for j in pool]
     t_add = time.perf_counter() - t
@@ -210,13 +321,16 @@ def bench_serving(args):
             h = srv.submit_refine(render_of[r.job], r.spp)
         handles.append(h)
     log(f"all {len(handles)} requests submitted after {time.perf_counter() - t0:.1f} s")
-    lat, paths, failed = [], 0, 0
+    lat, lat_render, lat_refine, paths, failed = [], [], [], 0, 0
+    kinds = [r.kind for r in reqs]
     for i, h in enumerate(handles):
-        info = srv.wait(h, timeout=args.drain_timeout)
-        if not info.ok:
+        try:
+            info = srv.wait(h, timeout=args.drain_timeout)
+        except oxpt.Error:
             failed += 1
             continue
         lat.append(info.latency_s)
+        (lat_render if kinds[i] == "render" else lat_refine).append(info.latency_s)
         paths += info.paths
         if (i + 1) % 100 == 0:
             log(f"{i + 1}/{len(handles)} requests done")
@@ -224,8 +338,9 @@ def bench_serving(args):
     st = srv.stats()
     lat_ms = sorted(x * 1e3 for x in lat)
 
-    def pct(q):
-        return round(lat_ms[min(len(lat_ms) - 1, int(round(q / 100.0 * (len(lat_ms) - 1))))], 3) if lat_ms else None
+    def pct(q, xs=None):
+        xs = lat_ms if xs is None else sorted(x * 1e3 for x in xs)
+        return round(xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))], 3) if xs else None
 
     busy = st.busy_s
     out = {
@@ -235,15 +350,20 @@ def bench_serving(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene generators)",
         "config": {"workload": f"serving: Poisson {args.rate} req/s for {args.duration:.0f} s from the uneven-batch "
                                f"distribution (64-scene pool) + refine(4) on 25% of renders; HBM budget "
-                               f"{args.budget_gb} GB; {args.spp_tick} spp per tick",
+                               f"{args.budget_gb} GB; " + (f"ticks of ~{args.tick_ms} ms, shortest remaining first"
+                                                           if args.tick_ms else f"{args.spp_tick} spp per tick"),
                    "scene_preparation": "device, at first use" if args.device_build else "host, at registration",
                    "registration_s": round(t_add, 3),
                    "requests": len(handles), "failed": failed, "offered_paths": int(offered),
                    "offered_mpaths_s": round(offered / args.duration / 1e6, 3)},
-        "latency_ms": {"p50": pct(50), "p99": pct(99)},
+        "latency_ms": {"p50": pct(50), "p90": pct(90), "p99": pct(99), "max": pct(100),
+                       "render": {"p50": pct(50, lat_render), "p99": pct(99, lat_render)},
+                       "refine": {"p50": pct(50, lat_refine), "p99": pct(99, lat_refine)}},
         "busy_mpaths_s": round(st.paths / busy / 1e6, 3) if busy > 0 else None,
         "server": {"ticks": st.ticks, "uploads": st.uploads, "evictions": st.evictions, "busy_s": round(busy, 3),
-                   "wall_s": round(t1 - t0, 3)},
+                   "io_s": round(st.io_s, 3), "failed_ticks": st.failed_ticks, "wall_s": round(t1 - t0, 3),
+                   "arrivals_s": round(args.duration, 3), "tick_ms": args.tick_ms,
+                   "spp_per_tick": args.spp_tick},
     }
     print(json.dumps(out), flush=True)
 
@@ -357,7 +477,9 @@ def main():
     ap.add_argument("--rate", type=float, default=50.0, help="c4: Poisson arrival rate (req/s)")
     ap.add_argument("--duration", type=float, default=60.0, help="c4: arrival window (s)")
     ap.add_argument("--budget-gb", type=int, default=200, help="c4: HBM admission budget")
-    ap.add_argument("--spp-tick", type=int, default=16, help="c4: samples per request per tick")
+    ap.add_argument("--spp-tick", type=int, default=0,
+                    help="c4: samples per request per tick (with --tick-ms: a cap, 0 = none)")
+    ap.add_argument("--tick-ms", type=int, default=10, help="c4: target tick duration (0: fixed --spp-tick)")
     ap.add_argument("--device-build", action="store_true", help="c4: prepare pool scenes on the device at first use")
     ap.add_argument("--drain-timeout", type=float, default=600.0, help="c4: wait per request (s)")
     ap.add_argument("--cpu-tiles", type=int, default=16)
@@ -452,45 +574,62 @@ def main():
 
         pk = oxpt.Packed(scene, opts, env)
         osc = oracle.Scene.from_packed(pk)
-        threads = min(16, os.cpu_count() or 1)
-        tl = tiles(w, h, args.cpu_tiles)
-        # Algorithmic bytes per path: instrumented run over two tiles.
+        threads = cpu_threads()
+        tl = tiles(w, h, args.cpu_tiles) if args.workload != "c0" else [(0, 0, w, h)]
+        cpu_paths = sum((r[2] - r[0]) * (r[3] - r[1]) for r in tl) * spp
+        # Algorithmic bytes per path: an instrumented pass over every tile
+        # (it also warms the CPU caches before the timed runs).
         cacc = np.zeros((h, w, 4), dtype=np.float32)
         cnt_paths = cnt_bytes = 0
-        for r in tl[:2]:
+        for r in tl:
             _, cnt = osc.render(0, spp, region=r, accum=cacc, threads=threads, counts=True)
             cnt_paths += cnt.paths
             cnt_bytes += cnt.bytes
         bytes_per_path = cnt_bytes / max(cnt_paths, 1)
-        log(f"oracle: {bytes_per_path:.0f} algorithmic bytes/path")
-        # Timed CPU baseline over every tile at full spp.
-        acc = np.zeros((h, w, 4), dtype=np.float32)
-        tc = time.perf_counter()
-        for r in tl:
-            osc.render(0, spp, region=r, accum=acc, threads=threads)
-        cpu_s = time.perf_counter() - tc
-        cpu_paths = sum((r[2] - r[0]) * (r[3] - r[1]) for r in tl) * spp
-        log(f"oracle: {cpu_paths} paths in {cpu_s:.2f} s on {threads} threads")
+        log(f"oracle: {bytes_per_path:.0f} algorithmic bytes/path over {len(tl)} tiles")
+        # Timed CPU baseline: median of 5 runs after the warm-up pass (as the
+        # reference's pt_bench times its renders), every tile at full spp.
+        runs = []
+        for k in range(5):
+            acc = np.zeros((h, w, 4), dtype=np.float32)
+            tc = time.perf_counter()
+            for r in tl:
+                osc.render(0, spp, region=r, accum=acc, threads=threads)
+            runs.append(time.perf_counter() - tc)
+            log(f"oracle run {k + 1}/5: {cpu_paths} paths in {runs[-1]:.2f} s on {threads} threads")
+        cpu_s = sorted(runs)[2]
         rp = oxpt.ResolveParams((1, 3, 0, 0), (1.0, float(spp), float(w), 0.0),
                                 tuple(c / 255.0 for c in opts.background))
         cimg = oracle.resolve(acc, rp)
         gt = np.concatenate([img[r[1]:r[3], r[0]:r[2]].reshape(-1, 4) for r in tl])
         ct = np.concatenate([cimg[r[1]:r[3], r[0]:r[2]].reshape(-1, 4) for r in tl])
         psnr_db = psnr(gt, ct)
+        sample = (f"{len(tl)} seeded 64x64 tiles of the same frame at {spp} spp ({cpu_paths} paths)"
+                  if args.workload != "c0" else f"the full frame at {spp} spp ({cpu_paths} paths)")
         cpu_baseline = {
             "value": round(cpu_paths / cpu_s / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"{len(tl)} seeded 64x64 tiles of the same frame at {spp} spp ({cpu_paths} paths), "
-                      "CPU oracle (C++ restatement of the reference estimator), std::thread row tiles",
+            "sample": sample + ", CPU oracle (C++ restatement of the reference estimator), std::thread row "
+                      "tiles, median of 5 runs after a warm-up",
+            "cpu_model": cpu_model(), "host_cpus_visible": os.cpu_count(),
         }
         achieved = bytes_per_path * per_launch_paths / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+        measured = pmc_measured(args)
+        cache_resident = args.workload == "c0"
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            # A working set of a few KB lives in the caches: an HBM fraction
+            # of it would mean nothing.
+            "frac": None if cache_resident else round(achieved / HBM_PEAK_GBS, 4),
             "traffic": pmc_traffic(args, per_launch_paths),
+            "model": "no-cache algorithmic bytes per path (instrumented oracle, every tile) x paths per launch / "
+                     "the launch's HIP-event time",
             "kernel": kernel_name,
             "bytes_per_path": round(bytes_per_path, 1), "paths_per_launch": int(per_launch_paths),
             "avg_launch_ms": round(avg_launch_s * 1e3, 3),
         }
+        if measured is not None:
+            roofline["measured"] = measured
+            roofline["limiter"] = measured.get("limiter")
     lat_ms = sorted(x * 1e3 for x in lat)
 
     def pct(p):

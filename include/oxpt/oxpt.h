@@ -313,6 +313,16 @@ int oxpt_ctx_get_tuning(const oxpt_ctx *ctx, oxpt_tuning *t);
  * returns it (and its trial time). All forms give bit-identical images. */
 int oxpt_autotune(oxpt_ctx *ctx, const oxpt_scene *scene, const oxpt_params *params, uint32_t spp_per_trial,
                   oxpt_tuning *best, double *best_ms);
+/* The batched launch's autotuner: times the caller's items (at most
+ * spp_per_trial samples each, into scratch accumulators) under the
+ * megakernel occupancy forms, samples per dispatch (max_paths 2^30, 2^28,
+ * 2^26: how many rounds the batch splits into) and traversal-stack LDS
+ * shares, and keeps the fastest in ctx's tuning. Same images for every
+ * candidate. No counterpart in the reference (its dispatch sizes,
+ * PATHS_PER_DISPATCH / PATHS_PER_SUBMIT at src/pathtrace.rs:85-95, are
+ * constants). */
+int oxpt_autotune_batch(oxpt_ctx *ctx, const oxpt_batch_item *items, size_t n, uint32_t spp_per_trial,
+                        oxpt_tuning *best, double *best_ms);
 
 /* Debug / parity probes (no counterpart in the reference: they expose the
  * kernel's integer sampler and light-CDF picks for bit-exact checks).

@@ -1592,6 +1592,70 @@ int oxpt_autotune(oxpt_ctx *c, const oxpt_scene *s, const oxpt_params *P, uint32
     return OXPT_OK;
 }
 
+int oxpt_autotune_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n, uint32_t spp_per_trial,
+                        oxpt_tuning *best_out, double *best_ms) {
+    if (!c || !items || n == 0) return OXPT_E_ARG;
+    if (setdev(c)) return OXPT_E_BACKEND;
+    // Trials run the caller's items with at most spp_per_trial samples each
+    // into scratch accumulators (the caller's stay untouched). Candidates:
+    // megakernel occupancy forms x samples per dispatch (the path cap that
+    // splits the batch into rounds) x traversal-stack LDS share. Every
+    // candidate gives the same images.
+    const uint32_t spp = std::max(1u, std::min(spp_per_trial, 256u));
+    std::vector<oxpt_accum *> tmp(n, nullptr);
+    std::vector<oxpt_batch_item> trial(items, items + n);
+    int rc = OXPT_OK;
+    for (size_t i = 0; i < n && rc == OXPT_OK; i++) {
+        if (!items[i].accum || !same_domain(items[i].accum->ctx, c)) rc = fail(c, OXPT_E_ARG, "autotune: bad item");
+        else rc = oxpt_accum_create(c, items[i].accum->w, items[i].accum->h, &tmp[i]);
+        if (rc == OXPT_OK) {
+            trial[i].accum = tmp[i];
+            trial[i].first = 0;
+            trial[i].count = std::min(items[i].count, spp);
+        }
+    }
+    struct Cand {
+        uint32_t kernel, max_paths, stack_lds;
+    };
+    std::vector<Cand> cands;
+    for (uint32_t k : {0u, 2u, 3u, 4u, 5u})
+        for (uint32_t mp : {1u << 30, 1u << 28, 1u << 26}) cands.push_back({k, mp, 0u});
+    cands.push_back({0u, 1u << 30, 8u});
+    cands.push_back({0u, 1u << 30, OXPT_STACK_PRIVATE});
+    const oxpt_tuning saved = c->tuning;
+    oxpt_tuning best = saved;
+    double best_t = 1e300;
+    for (const Cand &cd : cands) {
+        if (rc) break;
+        oxpt_tuning t = saved;
+        t.kernel = cd.kernel;
+        t.max_paths = cd.max_paths;
+        t.stack_lds = cd.stack_lds;
+        if ((rc = oxpt_ctx_set_tuning(c, &t))) break;
+        // One warm trial, then the faster of two timed ones (wall time of
+        // the whole call: rounds, uploads and launches).
+        if ((rc = oxpt_refine_batch(c, trial.data(), n))) break;
+        double cand = 1e300;
+        for (int rep = 0; rep < 2 && rc == OXPT_OK; rep++) {
+            auto t0 = std::chrono::steady_clock::now();
+            rc = oxpt_refine_batch(c, trial.data(), n);
+            if (rc == OXPT_OK) rc = oxpt_finish(c);
+            cand = std::min(cand, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        if (rc) break;
+        if (cand < best_t) {
+            best_t = cand;
+            best = c->tuning;
+        }
+    }
+    for (oxpt_accum *a : tmp) oxpt_accum_destroy(a);
+    c->tuning = rc ? saved : best;
+    if (rc) return rc;
+    if (best_out) *best_out = best;
+    if (best_ms) *best_ms = best_t;
+    return OXPT_OK;
+}
+
 int oxpt_ctx_timing(oxpt_ctx *c, int reset, uint64_t *launches, double *total_ms) {
     if (!c) return OXPT_E_ARG;
     if (setdev(c)) return OXPT_E_BACKEND;

@@ -190,6 +190,8 @@ def _sig(name, res, *args):
 # Layer 1.
 _sig("oxpt_ctx_create", C.c_int, C.c_int, P(_vp))
 _sig("oxpt_ctx_create_peer", C.c_int, _vp, P(_vp))
+_sig("oxpt_refine_batch_noalloc", C.c_int, _vp, _vp, sz)
+_sig("oxpt_autotune_batch", C.c_int, _vp, _vp, sz, u32, P(Tuning), P(C.c_double))
 _sig("oxpt_ctx_destroy", None, _vp)
 _sig("oxpt_last_error", C.c_char_p, _vp)
 _sig("oxpt_ctx_summary", C.c_char_p, _vp)
@@ -795,6 +797,26 @@ class Device:
         for i, (sc, acc, prm, first, count) in enumerate(items):
             arr[i] = BatchItem(sc.h, acc.h, prm, first, count)
         self._check(lib.oxpt_refine_batch(self.h, arr, len(items)), "refine_batch")
+
+    def refine_batch_noalloc(self, items) -> None:
+        """oxpt_refine_batch_noalloc: the same, with no device allocation."""
+        arr = (BatchItem * len(items))()
+        for i, (sc, acc, prm, first, count) in enumerate(items):
+            arr[i] = BatchItem(sc.h, acc.h, prm, first, count)
+        self._check(lib.oxpt_refine_batch_noalloc(self.h, arr, len(items)), "refine_batch_noalloc")
+
+    def autotune_batch(self, items, spp_per_trial: int = 16) -> Tuple[Tuning, float]:
+        """Pick the fastest form / samples per dispatch / stack split for these items (oxpt_autotune_batch)."""
+        arr = (BatchItem * len(items))()
+        for i, (sc, acc, prm, first, count) in enumerate(items):
+            arr[i] = BatchItem(sc.h, acc.h, prm, first, count)
+        t, ms = Tuning(), C.c_double()
+        self._check(lib.oxpt_autotune_batch(self.h, arr, len(items), spp_per_trial, C.byref(t), C.byref(ms)),
+                    "autotune_batch")
+        return t, ms.value
+
+    def last_kernel(self) -> str:
+        return lib.oxpt_ctx_last_kernel(self.h).decode()
 
     def finish(self) -> None:
         self._check(lib.oxpt_finish(self.h), "finish")
