@@ -630,7 +630,13 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
         const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
         const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
         const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
-        // Sort keys: (|t| bits << 32 | child) for hit children, all ones for misses.
+        // Sort keys: (|t| bits << 32 | child record) for hit children, all
+        // ones for misses. A key is the stack entry itself (stack_entry's
+        // short form: every wide child record fits, first < 2^27, count <=
+        // 30), so a push stores it and an entry reads its record back
+        // without selecting among the children. Equal entry distances are
+        // ordered by record, which only decides ties between exactly equal
+        // triangle distances, as any visiting order does.
         uint64_t k[4];
         {
             const float t0 = cn.x != WIDE_EMPTY ? slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f),
@@ -641,10 +647,10 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
                                                        make_float4(hx.z, hy.z, hz.z, 0.0f), t_max) : -1.0f;
             const float t3 = cn.w != WIDE_EMPTY ? slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f),
                                                        make_float4(hx.w, hy.w, hz.w, 0.0f), t_max) : -1.0f;
-            k[0] = t0 >= 0.0f ? ((uint64_t)(fbits(t0) & 0x7fffffffu) << 32) | 0u : ~0ull;
-            k[1] = t1 >= 0.0f ? ((uint64_t)(fbits(t1) & 0x7fffffffu) << 32) | 1u : ~0ull;
-            k[2] = t2 >= 0.0f ? ((uint64_t)(fbits(t2) & 0x7fffffffu) << 32) | 2u : ~0ull;
-            k[3] = t3 >= 0.0f ? ((uint64_t)(fbits(t3) & 0x7fffffffu) << 32) | 3u : ~0ull;
+            k[0] = t0 >= 0.0f ? ((uint64_t)(fbits(t0) & 0x7fffffffu) << 32) | (rf.x << 5 | cn.x) : ~0ull;
+            k[1] = t1 >= 0.0f ? ((uint64_t)(fbits(t1) & 0x7fffffffu) << 32) | (rf.y << 5 | cn.y) : ~0ull;
+            k[2] = t2 >= 0.0f ? ((uint64_t)(fbits(t2) & 0x7fffffffu) << 32) | (rf.z << 5 | cn.z) : ~0ull;
+            k[3] = t3 >= 0.0f ? ((uint64_t)(fbits(t3) & 0x7fffffffu) << 32) | (rf.w << 5 | cn.w) : ~0ull;
         }
         if (AnyUnordered && any) {
             // Any-hit query: t_max stays put until the walk ends at the
@@ -656,16 +662,13 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 if (k[j] == ~0ull) continue;
-                const uint32_t cref = j == 0 ? rf.x : j == 1 ? rf.y : j == 2 ? rf.z : rf.w;
-                const uint32_t ccnt = j == 0 ? cn.x : j == 1 ? cn.y : j == 2 ? cn.z : cn.w;
                 if (!entered) {
-                    first = cref;
-                    count = ccnt;
+                    first = (uint32_t)k[j] >> 5;
+                    count = (uint32_t)k[j] & 31u;
                     entered = true;
                 } else if (sp < WIDE_STACK) {
-                    const uint64_t e = (k[j] & 0xffffffff00000000ull) | (cref << 5 | ccnt);
-                    if (sp < in_lds) lds[sp * 64u] = e;
-                    else priv[sp - priv_off] = e;
+                    if (sp < in_lds) lds[sp * 64u] = k[j];
+                    else priv[sp - priv_off] = k[j];
                     sp++;
                 }
             }
@@ -685,19 +688,14 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
 #pragma unroll
             for (int j = 3; j >= 1; j--) {
                 if (k[j] == ~0ull) continue;
-                const uint32_t c = (uint32_t)k[j] & 3u;
-                const uint32_t cref = c == 0 ? rf.x : c == 1 ? rf.y : c == 2 ? rf.z : rf.w;
-                const uint32_t ccnt = c == 0 ? cn.x : c == 1 ? cn.y : c == 2 ? cn.z : cn.w;
                 if (sp < WIDE_STACK) {
-                    const uint64_t e = (k[j] & 0xffffffff00000000ull) | (cref << 5 | ccnt);
-                    if (sp < in_lds) lds[sp * 64u] = e;
-                    else priv[sp - priv_off] = e;
+                    if (sp < in_lds) lds[sp * 64u] = k[j];
+                    else priv[sp - priv_off] = k[j];
                     sp++;
                 }
             }
-            const uint32_t c = (uint32_t)k[0] & 3u;
-            first = c == 0 ? rf.x : c == 1 ? rf.y : c == 2 ? rf.z : rf.w;
-            count = c == 0 ? cn.x : c == 1 ? cn.y : c == 2 ? cn.z : cn.w;
+            first = (uint32_t)k[0] >> 5;
+            count = (uint32_t)k[0] & 31u;
             return true;
         }
         }
