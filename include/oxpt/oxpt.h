@@ -46,6 +46,7 @@ extern "C" {
 typedef struct oxpt_ctx oxpt_ctx;
 typedef struct oxpt_scene oxpt_scene;
 typedef struct oxpt_accum oxpt_accum;
+typedef struct oxpt_image oxpt_image;
 
 /* Packed scene words (include/oxpt/layout.h), copied at upload. Replaces the
  * buffer creation in GpuPathTracer::upload / ensure_binding
@@ -236,6 +237,20 @@ int oxpt_resolve_device(oxpt_ctx *ctx, const oxpt_accum *acc, const oxpt_resolve
 
 /* Kernel timing: HIP events around every trace launch on the context's
  * stream. Returns launches and total milliseconds since the last reset. */
+/* A display image the library owns (the reference's tracer-owned output
+ * texture, src/pathtrace.rs:687-691): w x h RGBA8 (hdr = 0) or RGBA32F rows
+ * in device memory. oxpt_image_resolve fills it from an accumulator like
+ * oxpt_resolve_device (asynchronous; finish before another stream or
+ * process reads it). oxpt_image_export writes a 64-byte HIP IPC handle
+ * (hipIpcMemHandle_t) a viewer process opens with hipIpcOpenMemHandle to
+ * read the same memory without a copy; images of 2 GiB or more are not
+ * exported. */
+int oxpt_image_create(oxpt_ctx *ctx, uint32_t w, uint32_t h, int hdr, oxpt_image **out);
+void oxpt_image_destroy(oxpt_image *image);
+void *oxpt_image_device_ptr(const oxpt_image *image);
+size_t oxpt_image_bytes(const oxpt_image *image);
+int oxpt_image_resolve(oxpt_ctx *ctx, const oxpt_accum *acc, const oxpt_resolve_params *rp, oxpt_image *image);
+int oxpt_image_export(oxpt_ctx *ctx, const oxpt_image *image, void *handle_out, size_t handle_bytes);
 int oxpt_ctx_timing(oxpt_ctx *ctx, int reset, uint64_t *launches, double *total_ms);
 
 /* Batched refine (many scenes, cameras and sample runs of uneven size in one
@@ -490,6 +505,14 @@ int oxpt_tracer_hdr(oxpt_tracer *t, float *out_rgba32f, size_t bytes);
  * with the current options into device memory (oxpt_resolve_device): RGBA8
  * (hdr = 0, w*h*4 bytes) or RGBA32F (hdr = 1); asynchronous. */
 int oxpt_tracer_draw(oxpt_tracer *t, void *device_out, size_t bytes, int hdr);
+/* draw_texture with the tracer's own output (src/pathtrace.rs:687-691, which
+ * returns the tracer's texture): resolves into an image the tracer owns
+ * (made or remade when the size or format changes) and returns it; valid
+ * until the next draw_texture of another size or format, or oxpt_tracer_free.
+ * Asynchronous: oxpt_tracer_finish before reading it elsewhere. */
+int oxpt_tracer_draw_texture(oxpt_tracer *t, int hdr, oxpt_image **out);
+/* The accumulator of the current accumulation (NULL before the first sync). */
+const oxpt_accum *oxpt_tracer_accum(oxpt_tracer *t);
 void oxpt_tracer_reset(oxpt_tracer *t);
 uint32_t oxpt_tracer_samples(const oxpt_tracer *t);
 int oxpt_tracer_is_converged(const oxpt_tracer *t);

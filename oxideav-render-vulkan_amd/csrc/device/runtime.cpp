@@ -177,6 +177,14 @@ struct oxpt_accum {
     size_t bytes = 0;
 };
 
+struct oxpt_image {
+    oxpt_ctx *ctx = nullptr;
+    void *d = nullptr;
+    uint32_t w = 0, h = 0;
+    int hdr = 0;
+    size_t bytes = 0;
+};
+
 namespace {
 
 int fail(oxpt_ctx *c, int code, const std::string &msg) {
@@ -1757,6 +1765,56 @@ int copy_to_host(oxpt_ctx *c, void *host, const void *dev, size_t bytes) {
 }  // namespace oxpt
 
 extern "C" {
+
+int oxpt_image_create(oxpt_ctx *c, uint32_t w, uint32_t h, int hdr, oxpt_image **out) {
+    if (!c || !out) return OXPT_E_ARG;
+    *out = nullptr;
+    if (w == 0 || h == 0) return fail(c, OXPT_E_INVALID, "image: zero dimension");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    auto *im = new oxpt_image();
+    im->ctx = c->base;
+    im->w = w;
+    im->h = h;
+    im->hdr = hdr ? 1 : 0;
+    im->bytes = (size_t)w * h * (hdr ? 16u : 4u);
+    if (int rc = dalloc(c, im->bytes, &im->d, "display image")) {
+        delete im;
+        return rc;
+    }
+    *out = im;
+    return OXPT_OK;
+}
+
+void oxpt_image_destroy(oxpt_image *im) {
+    if (!im) return;
+    setdev(im->ctx);
+    (void)hipStreamSynchronize(im->ctx->stream);
+    dfree(im->ctx, im->d, im->bytes);
+    delete im;
+}
+
+void *oxpt_image_device_ptr(const oxpt_image *im) { return im ? im->d : nullptr; }
+size_t oxpt_image_bytes(const oxpt_image *im) { return im ? im->bytes : 0; }
+
+int oxpt_image_resolve(oxpt_ctx *c, const oxpt_accum *a, const oxpt_resolve_params *rp, oxpt_image *im) {
+    if (!c || !a || !rp || !im || !same_domain(a->ctx, c) || !same_domain(im->ctx, c)) return OXPT_E_ARG;
+    if (im->w != a->w || im->h != a->h || im->hdr != (rp->cfg[3] == 1 ? 1 : 0))
+        return fail(c, OXPT_E_ARG, "image: size or format differs from the resolve");
+    return oxpt_resolve_device(c, a, rp, im->d, im->bytes);
+}
+
+int oxpt_image_export(oxpt_ctx *c, const oxpt_image *im, void *handle_out, size_t handle_bytes) {
+    if (!c || !im || !handle_out || !same_domain(im->ctx, c)) return OXPT_E_ARG;
+    if (handle_bytes < sizeof(hipIpcMemHandle_t)) return fail(c, OXPT_E_ARG, "image export: handle buffer too small");
+    // Shared allocations are kept under 2 GiB (a larger one is refused).
+    if (im->bytes >= (2ull << 30)) return fail(c, OXPT_E_ARG, "image export: images of 2 GiB or more are not shared");
+    if (setdev(c)) return OXPT_E_BACKEND;
+    hipIpcMemHandle_t h;
+    hipError_t e = hipIpcGetMemHandle(&h, im->d);
+    if (e != hipSuccess) return hip_fail(c, "image export", e);
+    std::memcpy(handle_out, &h, sizeof h);
+    return OXPT_OK;
+}
 
 int oxpt_debug_pattern_words(oxpt_ctx *c, const oxpt_scene *s, uint32_t seed, size_t n, const uint32_t *x,
                              const uint32_t *y, const uint32_t *index, const uint32_t *p, uint32_t *out4) {

@@ -221,7 +221,11 @@ struct oxpt_tracer {
         int rc = device ? oxpt_resolve_device(ctx, accum, &rp, out, bytes) : oxpt_resolve(ctx, accum, &rp, out, bytes);
         return rc ? dev_fail(rc) : OXPT_OK;
     }
+    oxpt_image *texture = nullptr;  // draw_texture output, tex_w x tex_h, tex_hdr
+    uint32_t tex_w = 0, tex_h = 0;
+    int tex_hdr = 0;
     ~oxpt_tracer() {
+        if (texture) oxpt_image_destroy(texture);
         if (accum) oxpt_accum_destroy(accum);
         if (dscene) oxpt_scene_destroy(dscene);
         if (ctx) oxpt_ctx_destroy(ctx);
@@ -379,6 +383,29 @@ int oxpt_tracer_hdr(oxpt_tracer *t, float *out, size_t bytes) { return t ? t->re
 int oxpt_tracer_draw(oxpt_tracer *t, void *device_out, size_t bytes, int hdr) {
     return t ? t->resolve(hdr != 0, device_out, bytes, true) : OXPT_E_ARG;
 }
+int oxpt_tracer_draw_texture(oxpt_tracer *t, int hdr, oxpt_image **out) {
+    if (!t || !out) return OXPT_E_ARG;
+    *out = nullptr;
+    // An accumulator exists after the first resolve (resolve makes a
+    // background frame when never synced); the image follows its size.
+    uint32_t ww = std::max(t->w, 1u), hh = std::max(t->h, 1u);
+    const size_t want = (size_t)ww * hh * (hdr ? 16u : 4u);
+    if (t->texture && (t->tex_w != ww || t->tex_h != hh || t->tex_hdr != (hdr ? 1 : 0))) {
+        oxpt_finish(t->ctx);
+        oxpt_image_destroy(t->texture);
+        t->texture = nullptr;
+    }
+    if (!t->texture) {
+        if (int rc = oxpt_image_create(t->ctx, ww, hh, hdr, &t->texture)) return t->dev_fail(rc);
+        t->tex_w = ww;
+        t->tex_h = hh;
+        t->tex_hdr = hdr ? 1 : 0;
+    }
+    if (int rc = t->resolve(hdr != 0, oxpt_image_device_ptr(t->texture), want, true)) return rc;
+    *out = t->texture;
+    return OXPT_OK;
+}
+const oxpt_accum *oxpt_tracer_accum(oxpt_tracer *t) { return t ? t->accum : nullptr; }
 void oxpt_tracer_reset(oxpt_tracer *t) {
     if (t) t->reset();
 }

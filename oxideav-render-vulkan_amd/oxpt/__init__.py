@@ -190,6 +190,14 @@ def _sig(name, res, *args):
 # Layer 1.
 _sig("oxpt_ctx_create", C.c_int, C.c_int, P(_vp))
 _sig("oxpt_ctx_create_peer", C.c_int, _vp, P(_vp))
+_sig("oxpt_image_create", C.c_int, _vp, u32, u32, C.c_int, P(_vp))
+_sig("oxpt_image_destroy", None, _vp)
+_sig("oxpt_image_device_ptr", _vp, _vp)
+_sig("oxpt_image_bytes", sz, _vp)
+_sig("oxpt_image_resolve", C.c_int, _vp, _vp, P(ResolveParams), _vp)
+_sig("oxpt_image_export", C.c_int, _vp, _vp, _vp, sz)
+_sig("oxpt_tracer_draw_texture", C.c_int, _vp, C.c_int, P(_vp))
+_sig("oxpt_tracer_accum", _vp, _vp)
 _sig("oxpt_refine_batch_noalloc", C.c_int, _vp, _vp, sz)
 _sig("oxpt_autotune_batch", C.c_int, _vp, _vp, sz, u32, P(Tuning), P(C.c_double))
 _sig("oxpt_ctx_destroy", None, _vp)
@@ -612,6 +620,26 @@ class GpuPathTracer:
         RGBA32F rows), asynchronously; finish() orders host reads after it."""
         self._check(lib.oxpt_tracer_draw(self.h, _vp(device_ptr), nbytes, 1 if hdr else 0), "draw")
 
+    def draw_texture(self, hdr: bool = False) -> "Image":
+        """draw_texture into the tracer's own image (resolved asynchronously;
+        finish() before reading it elsewhere). The image stays the tracer's."""
+        h = _vp()
+        self._check(lib.oxpt_tracer_draw_texture(self.h, 1 if hdr else 0, C.byref(h)), "draw_texture")
+        w, hh = self.size()
+        return Image(h.value, self.ctx, max(w, 1), max(hh, 1), hdr)
+
+    def accum(self) -> np.ndarray:
+        """The current accumulator (float4 per pixel: rgb sums, covered count)."""
+        a = lib.oxpt_tracer_accum(self.h)
+        if not a:
+            raise Error("accum: not synced")
+        w, h = self.size()
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        rc = lib.oxpt_accum_read(self.ctx, a, _ptr(out, f32), out.nbytes)
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        return out
+
     def set_device_build(self, enable: bool = True) -> None:
         """Prepare scenes on the device (BVH, triangle / attribute words,
         4-wide nodes) from the next sync on; the words are the host packer's."""
@@ -733,6 +761,37 @@ def _hip_runtime():
         _hip.hipMemcpy.argtypes = [_vp, _vp, sz, C.c_int]
         _hip.hipSetDevice.argtypes = [C.c_int]
     return _hip
+
+
+class Image:
+    """A display image the library owns (oxpt_image): device rows of RGBA8 or
+    RGBA32F. Not freed here: a tracer's draw_texture image belongs to the tracer."""
+
+    def __init__(self, handle: int, ctx: int, w: int, h: int, hdr: bool):
+        self.h, self.ctx, self.w, self.hgt, self.hdr = handle, ctx, w, h, hdr
+
+    @property
+    def ptr(self) -> int:
+        return lib.oxpt_image_device_ptr(self.h)
+
+    @property
+    def nbytes(self) -> int:
+        return lib.oxpt_image_bytes(self.h)
+
+    def read(self) -> np.ndarray:
+        """Device -> host copy (after the producer's finish)."""
+        out = np.empty((self.hgt, self.w, 4), dtype=np.float32 if self.hdr else np.uint8)
+        if _hip_runtime().hipMemcpy(out.ctypes.data_as(_vp), _vp(self.ptr), out.nbytes, 2) != 0:
+            raise BackendError("hipMemcpy failed")
+        return out
+
+    def export(self) -> bytes:
+        """The 64-byte HIP IPC handle another process opens with hipIpcOpenMemHandle."""
+        buf = (C.c_uint8 * 64)()
+        rc = lib.oxpt_image_export(self.ctx, self.h, buf, 64)
+        if rc:
+            _raise(rc, lib.oxpt_last_error(self.ctx).decode())
+        return bytes(buf)
 
 
 class DeviceBuffer:
