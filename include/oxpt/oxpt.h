@@ -451,6 +451,11 @@ void oxpt_material_default(oxpt_material *m);
  * the estimator (each slot picks its colour space). Returns the index. */
 int oxpt_host_scene_add_texture(oxpt_host_scene *s, uint32_t w, uint32_t h, const uint8_t *rgba8,
                                 const oxpt_sampler *sampler);
+/* A texture named by uri, its pixels supplied at prepare time by the
+ * tracer's texture resolver (the reference's texture sources, decoded by its
+ * TextureResolver; src/pathtrace.rs:474-478, 1261-1263). Returns its index,
+ * as oxpt_host_scene_add_texture does. */
+int oxpt_host_scene_add_texture_ref(oxpt_host_scene *s, const char *uri, const oxpt_sampler *sampler);
 /* Append n triangles: positions 9 floats each; optional per-corner arrays
  * (normals 9, tangents 12, uv0 6, uv1 6, colors 12 floats per triangle;
  * NULL = absent); one material index per triangle. */
@@ -513,6 +518,15 @@ int oxpt_tracer_draw(oxpt_tracer *t, void *device_out, size_t bytes, int hdr);
 int oxpt_tracer_draw_texture(oxpt_tracer *t, int hdr, oxpt_image **out);
 /* The accumulator of the current accumulation (NULL before the first sync). */
 const oxpt_accum *oxpt_tracer_accum(oxpt_tracer *t);
+/* set_texture_resolver (src/pathtrace.rs:474-478, the Renderer trait's at
+ * :1261-1263): the decoder of textures named by uri. It returns 0 and the
+ * level-0 RGBA8 pixels (w x h x 4 bytes, read before it is called again) or
+ * nonzero when it cannot decode the uri; such a texture, or any named texture
+ * while no resolver is set, is absent (its material slots sample nothing, as
+ * unbound slots). Installing or removing a resolver drops the decoded images
+ * and re-prepares the scene on the next sync, which then resets. */
+typedef int (*oxpt_texture_resolver)(void *user, const char *uri, uint32_t *w, uint32_t *h, const uint8_t **rgba8);
+int oxpt_tracer_set_texture_resolver(oxpt_tracer *t, oxpt_texture_resolver fn, void *user);
 void oxpt_tracer_reset(oxpt_tracer *t);
 uint32_t oxpt_tracer_samples(const oxpt_tracer *t);
 int oxpt_tracer_is_converged(const oxpt_tracer *t);

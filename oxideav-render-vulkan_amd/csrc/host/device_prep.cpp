@@ -8,8 +8,10 @@
 
 namespace oxpt {
 
-int device_prepare(oxpt_ctx *ctx, const HostScene &s, Packed &pk, oxpt_scene **out, oxpt_build_info *info) {
-    pk = pack_scene(s, false, nullptr, false, false);
+int device_prepare(oxpt_ctx *ctx, const HostScene &s, Packed &pk, oxpt_scene **out, oxpt_build_info *info,
+                   const std::vector<Texture> *textures) {
+    pk = pack_scene(s, false, nullptr, false, false, textures);
+    const std::vector<Texture> &tex_list = textures ? *textures : s.textures;
     const Soup &so = s.soup;
     const std::vector<float> mat_lum = material_emission(s);
     oxpt_soup_desc d{};
@@ -28,9 +30,9 @@ int device_prepare(oxpt_ctx *ctx, const HostScene &s, Packed &pk, oxpt_scene **o
     d.atlas_layers = pk.atlas_layers;
     d.material_emission = mat_lum.data();
     d.n_materials = (uint32_t)mat_lum.size();
-    std::vector<oxpt_texture_desc> tex(s.textures.size());
-    for (size_t i = 0; i < s.textures.size(); i++)
-        tex[i] = {s.textures[i].w, s.textures[i].h, s.textures[i].rgba8.empty() ? nullptr : s.textures[i].rgba8.data()};
+    std::vector<oxpt_texture_desc> tex(tex_list.size());
+    for (size_t i = 0; i < tex_list.size(); i++)
+        tex[i] = {tex_list[i].w, tex_list[i].h, tex_list[i].rgba8.empty() ? nullptr : tex_list[i].rgba8.data()};
     float lut[512];
     texel_decode_lut(lut);
     oxpt_atlas_desc ad{tex.size(), tex.data(), pk.chains.size(), pk.chains.data(), lut};

@@ -334,8 +334,9 @@ CameraFrame resolve_camera(const CameraSource &s, uint32_t w, uint32_t h) {
 }
 
 Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *light_of_out, bool atlas,
-                  bool emissive) {
+                  bool emissive, const std::vector<Texture> *textures) {
     Packed pk;
+    const std::vector<Texture> &tex_list = textures ? *textures : s.textures;
     const Soup &soup = s.soup;
     size_t ntri = soup.count();
     // Without `emissive` the lights are found by the device builder, which
@@ -465,7 +466,7 @@ Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *ligh
     for (const auto &m : s.materials)
         for (uint32_t sl = 0; sl < OXPT_N_TEX_SLOTS; sl++) {
             int ti = m.tex[sl].texture;
-            if (ti >= 0 && (size_t)ti < s.textures.size() && s.textures[ti].w > 0) descs[{ti, kSlotSrgb[sl]}] = 0;
+            if (ti >= 0 && (size_t)ti < tex_list.size() && tex_list[ti].w > 0) descs[{ti, kSlotSrgb[sl]}] = 0;
         }
     std::vector<std::pair<int, bool>> keys;
     for (auto &kv : descs) keys.push_back(kv.first);
@@ -474,8 +475,8 @@ Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *ligh
     std::vector<std::vector<std::pair<uint32_t, uint32_t>>> dims;
     std::vector<std::vector<Mip>> chains;
     for (auto &k : keys) {
-        dims.push_back(mip_dims(s.textures[k.first].w, s.textures[k.first].h));
-        if (atlas) chains.push_back(build_mips(s.textures[k.first], k.second));
+        dims.push_back(mip_dims(tex_list[k.first].w, tex_list[k.first].h));
+        if (atlas) chains.push_back(build_mips(tex_list[k.first], k.second));
     }
     uint32_t biggest = 1;
     for (auto &c : dims) biggest = std::max(biggest, std::max(c[0].first, c[0].second));
@@ -537,7 +538,7 @@ Packed pack_scene(const HostScene &s, bool geometry, std::vector<uint32_t> *ligh
     for (size_t ci = 0; ci < keys.size(); ci++) {
         descs[keys[ci]] = (uint32_t)t.size();
         pk.chains.push_back({(uint32_t)keys[ci].first, keys[ci].second ? 1u : 0u, (uint32_t)t.size(), 0u});
-        const Texture &tx = s.textures[keys[ci].first];
+        const Texture &tx = tex_list[keys[ci].first];
         t.insert(t.end(), {placed[ci].levels, sampler_flags(tx.sampler), tx.w, tx.h, placed[ci].skip,
                            (uint32_t)placed[ci].rects.size(), 0u, 0u});
         for (const Rect &r : placed[ci].rects) t.insert(t.end(), {r.layer, r.x, r.y, r.w | (r.h << 16)});

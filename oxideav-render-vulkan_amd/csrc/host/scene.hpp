@@ -14,9 +14,10 @@
 namespace oxpt {
 
 struct Texture {
-    uint32_t w = 0, h = 0;
+    uint32_t w = 0, h = 0;       // 0 x 0: absent (its material slots sample nothing)
     std::vector<uint8_t> rgba8;  // level 0
     oxpt_sampler sampler{2, 2, 1, 5};
+    std::string uri;             // named texture: pixels come from a texture resolver
 };
 
 // Per-triangle record: positions plus optional per-corner attributes.
@@ -113,8 +114,10 @@ uint32_t build_wide(const std::vector<uint32_t> &node_words, std::vector<float> 
 // With emissive = false no emissive records are written (n_emissive = 0):
 // the device builder finds the lights from material_emission() and appends
 // the records at the end of the tables.
+// textures (when given) replaces s.textures: the resolved texture list of a
+// scene whose textures are named by uri (oxpt_tracer_set_texture_resolver).
 Packed pack_scene(const HostScene &s, bool geometry = true, std::vector<uint32_t> *light_of_out = nullptr,
-                  bool atlas = true, bool emissive = true);
+                  bool atlas = true, bool emissive = true, const std::vector<Texture> *textures = nullptr);
 // Per material: luminance of the emissive factor for emitting materials
 // (not unlit, some component > 0), else 0 (EmissiveLights::build's weight
 // per unit area).
@@ -136,7 +139,8 @@ std::vector<double> env_row_weights(uint32_t h);
 // Device scene preparation (oxpt_scene_build) of a host scene: pk receives
 // the host-packed tables with the device-found emissive records and the
 // geometry counts Params need; geom / attrs / atlas stay on the device.
-int device_prepare(oxpt_ctx *ctx, const HostScene &s, Packed &pk, oxpt_scene **out, oxpt_build_info *info);
+int device_prepare(oxpt_ctx *ctx, const HostScene &s, Packed &pk, oxpt_scene **out, oxpt_build_info *info,
+                   const std::vector<Texture> *textures = nullptr);
 // Upload the scene tables, with the environment section built on the
 // device after them when env is a full map (else the tables alone).
 int upload_tables(oxpt_ctx *ctx, oxpt_scene *dev, const std::vector<uint32_t> &scene_tables, const EnvMap *env);

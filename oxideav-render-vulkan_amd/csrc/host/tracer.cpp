@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <optional>
 #include <string>
@@ -76,7 +77,9 @@ struct oxpt_tracer {
             dscene = nullptr;
         }
         auto pk = std::make_unique<oxpt::Packed>();
-        if (int rc = oxpt::device_prepare(ctx, s, *pk, &dscene, &build_info)) return dev_fail(rc);
+        std::vector<oxpt::Texture> resolved;
+        const std::vector<oxpt::Texture> *tl = resolve_textures(s, resolved);
+        if (int rc = oxpt::device_prepare(ctx, s, *pk, &dscene, &build_info, tl)) return dev_fail(rc);
         packed = std::move(pk);
         words_on_device = true;
         cam_src = oxpt::camera_source(s);
@@ -108,7 +111,9 @@ struct oxpt_tracer {
         build_info = oxpt_build_info{};
         if (device_build && s.soup.count() > 0) return upload_device(s);
         words_on_device = false;
-        auto pk = std::make_unique<oxpt::Packed>(oxpt::pack_scene(s));
+        std::vector<oxpt::Texture> resolved;
+        const std::vector<oxpt::Texture> *tl = resolve_textures(s, resolved);
+        auto pk = std::make_unique<oxpt::Packed>(oxpt::pack_scene(s, true, nullptr, true, true, tl));
         if (dscene) {
             oxpt_scene_destroy(dscene);
             dscene = nullptr;
@@ -221,6 +226,45 @@ struct oxpt_tracer {
         int rc = device ? oxpt_resolve_device(ctx, accum, &rp, out, bytes) : oxpt_resolve(ctx, accum, &rp, out, bytes);
         return rc ? dev_fail(rc) : OXPT_OK;
     }
+    // set_texture_resolver: the resolver of textures named by uri, and the
+    // textures it decoded (kept until the resolver changes, as the
+    // reference's TextureCache keeps its decoded images).
+    oxpt_texture_resolver resolver = nullptr;
+    void *resolver_user = nullptr;
+    std::map<std::string, oxpt::Texture> tex_cache;
+    // The scene's texture list with every named texture resolved (absent,
+    // 0 x 0, when the resolver gives none); nullptr when no texture is named.
+    const std::vector<oxpt::Texture> *resolve_textures(const oxpt::HostScene &s, std::vector<oxpt::Texture> &out) {
+        bool named = false;
+        for (const auto &tx : s.textures) named = named || !tx.uri.empty();
+        if (!named) return nullptr;
+        out.clear();
+        out.reserve(s.textures.size());
+        for (const auto &tx : s.textures) {
+            if (tx.uri.empty()) {
+                out.push_back(tx);
+                continue;
+            }
+            auto it = tex_cache.find(tx.uri);
+            if (it == tex_cache.end()) {
+                oxpt::Texture dec;
+                uint32_t ww = 0, hh = 0;
+                const uint8_t *px = nullptr;
+                if (resolver && resolver(resolver_user, tx.uri.c_str(), &ww, &hh, &px) == 0 && px && ww > 0 && hh > 0 &&
+                    ww <= 65535 && hh <= 65535) {
+                    dec.w = ww;
+                    dec.h = hh;
+                    dec.rgba8.assign(px, px + (size_t)ww * hh * 4);
+                }
+                it = tex_cache.emplace(tx.uri, std::move(dec)).first;
+            }
+            oxpt::Texture r = it->second;
+            r.sampler = tx.sampler;
+            r.uri = tx.uri;
+            out.push_back(std::move(r));
+        }
+        return &out;
+    }
     oxpt_image *texture = nullptr;  // draw_texture output, tex_w x tex_h, tex_hdr
     uint32_t tex_w = 0, tex_h = 0;
     int tex_hdr = 0;
@@ -292,6 +336,16 @@ int oxpt_host_scene_add_texture(oxpt_host_scene *s, uint32_t w, uint32_t h, cons
     t.w = w;
     t.h = h;
     t.rgba8.assign(rgba8, rgba8 + (size_t)w * h * 4);
+    if (sampler) t.sampler = *sampler;
+    s->s.textures.push_back(std::move(t));
+    s->s.generation++;
+    return (int)s->s.textures.size() - 1;
+}
+
+int oxpt_host_scene_add_texture_ref(oxpt_host_scene *s, const char *uri, const oxpt_sampler *sampler) {
+    if (!s || !uri || !*uri) return OXPT_E_ARG;
+    oxpt::Texture t;
+    t.uri = uri;
     if (sampler) t.sampler = *sampler;
     s->s.textures.push_back(std::move(t));
     s->s.generation++;
@@ -406,6 +460,14 @@ int oxpt_tracer_draw_texture(oxpt_tracer *t, int hdr, oxpt_image **out) {
     return OXPT_OK;
 }
 const oxpt_accum *oxpt_tracer_accum(oxpt_tracer *t) { return t ? t->accum : nullptr; }
+int oxpt_tracer_set_texture_resolver(oxpt_tracer *t, oxpt_texture_resolver fn, void *user) {
+    if (!t) return OXPT_E_ARG;
+    t->resolver = fn;
+    t->resolver_user = user;
+    t->tex_cache.clear();
+    t->dirty = true;  // re-prepare on the next sync (src/pathtrace.rs:474-478)
+    return OXPT_OK;
+}
 void oxpt_tracer_reset(oxpt_tracer *t) {
     if (t) t->reset();
 }

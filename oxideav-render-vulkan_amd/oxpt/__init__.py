@@ -198,6 +198,9 @@ _sig("oxpt_image_resolve", C.c_int, _vp, _vp, P(ResolveParams), _vp)
 _sig("oxpt_image_export", C.c_int, _vp, _vp, _vp, sz)
 _sig("oxpt_tracer_draw_texture", C.c_int, _vp, C.c_int, P(_vp))
 _sig("oxpt_tracer_accum", _vp, _vp)
+TEXTURE_RESOLVER = C.CFUNCTYPE(C.c_int, _vp, C.c_char_p, P(u32), P(u32), P(P(C.c_uint8)))
+_sig("oxpt_tracer_set_texture_resolver", C.c_int, _vp, TEXTURE_RESOLVER, _vp)
+_sig("oxpt_host_scene_add_texture_ref", C.c_int, _vp, C.c_char_p, P(Sampler))
 _sig("oxpt_refine_batch_noalloc", C.c_int, _vp, _vp, sz)
 _sig("oxpt_autotune_batch", C.c_int, _vp, _vp, sz, u32, P(Tuning), P(C.c_double))
 _sig("oxpt_ctx_destroy", None, _vp)
@@ -407,6 +410,14 @@ class Scene:
         a = np.ascontiguousarray(rgba8, dtype=np.uint8)
         s = Sampler(*sampler)
         return lib.oxpt_host_scene_add_texture(self.h, a.shape[1], a.shape[0], _ptr(a, C.c_uint8), C.byref(s))
+
+    def add_texture_ref(self, uri: str, sampler: Tuple[int, int, int, int] = (2, 2, 1, 5)) -> int:
+        """A texture named by uri; a tracer's texture resolver supplies its pixels."""
+        sm = Sampler(*sampler)
+        i = lib.oxpt_host_scene_add_texture_ref(self.h, uri.encode(), C.byref(sm))
+        if i < 0:
+            _raise(i, "add_texture_ref")
+        return i
 
     def add_triangles(self, pos, material, normals=None, tangents=None, uv0=None, uv1=None, colors=None):
         pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 9)
@@ -619,6 +630,35 @@ class GpuPathTracer:
         """draw_texture: resolve into device memory at device_ptr (RGBA8 or
         RGBA32F rows), asynchronously; finish() orders host reads after it."""
         self._check(lib.oxpt_tracer_draw(self.h, _vp(device_ptr), nbytes, 1 if hdr else 0), "draw")
+
+    def set_texture_resolver(self, resolver: Optional[Callable[[str], Optional[np.ndarray]]]) -> None:
+        """set_texture_resolver: resolver(uri) -> (h, w, 4) uint8 RGBA8 level 0, or
+        None when it cannot decode uri. None removes it. The scene is re-prepared
+        (and the accumulation reset) on the next sync."""
+        if resolver is None:
+            self._resolver = None
+            self._check(lib.oxpt_tracer_set_texture_resolver(self.h, TEXTURE_RESOLVER(), None), "set_texture_resolver")
+            return
+        keep = {}
+
+        def trampoline(_user, uri, w, h, px):
+            try:
+                img = resolver(uri.decode())
+            except Exception:  # a failing decoder leaves the texture absent
+                return 1
+            if img is None:
+                return 1
+            a = np.ascontiguousarray(img, dtype=np.uint8)
+            if a.ndim != 3 or a.shape[2] != 4:
+                return 1
+            keep["last"] = a  # read by the tracer before the next call
+            w[0], h[0] = a.shape[1], a.shape[0]
+            px[0] = a.ctypes.data_as(P(C.c_uint8))
+            return 0
+
+        cb = TEXTURE_RESOLVER(trampoline)
+        self._resolver = (cb, keep)
+        self._check(lib.oxpt_tracer_set_texture_resolver(self.h, cb, None), "set_texture_resolver")
 
     def draw_texture(self, hdr: bool = False) -> "Image":
         """draw_texture into the tracer's own image (resolved asynchronously;
