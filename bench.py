@@ -69,6 +69,7 @@ def workload(name):
 PMC_FILES = {
     "c1": ("trace_persistent_c1_pmc.json", "trace_persistent<5,early>"),  # no punctual lights
     "c2": ("trace_persistent_c2_pmc.json", "trace_persistent<4,queue>"),  # 16 punctual lights
+    "c0": ("trace_batch_c0_pmc.json", "trace_batch<3,early>"),  # the one-job batch of a small frame
 }
 
 
