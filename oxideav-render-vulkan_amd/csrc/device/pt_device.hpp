@@ -630,50 +630,51 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
         const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
         const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
         const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
-        // Sort keys: (|t| bits << 32 | child record) for hit children, all
-        // ones for misses. A key is the stack entry itself (stack_entry's
-        // short form: every wide child record fits, first < 2^27, count <=
-        // 30), so a push stores it and an entry reads its record back
-        // without selecting among the children. Equal entry distances are
-        // ordered by record, which only decides ties between exactly equal
-        // triangle distances, as any visiting order does.
-        uint64_t k[4];
-        {
-            const float t0 = cn.x != WIDE_EMPTY ? slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f),
-                                                       make_float4(hx.x, hy.x, hz.x, 0.0f), t_max) : -1.0f;
-            const float t1 = cn.y != WIDE_EMPTY ? slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f),
-                                                       make_float4(hx.y, hy.y, hz.y, 0.0f), t_max) : -1.0f;
-            const float t2 = cn.z != WIDE_EMPTY ? slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f),
-                                                       make_float4(hx.z, hy.z, hz.z, 0.0f), t_max) : -1.0f;
-            const float t3 = cn.w != WIDE_EMPTY ? slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f),
-                                                       make_float4(hx.w, hy.w, hz.w, 0.0f), t_max) : -1.0f;
-            k[0] = t0 >= 0.0f ? ((uint64_t)(fbits(t0) & 0x7fffffffu) << 32) | (rf.x << 5 | cn.x) : ~0ull;
-            k[1] = t1 >= 0.0f ? ((uint64_t)(fbits(t1) & 0x7fffffffu) << 32) | (rf.y << 5 | cn.y) : ~0ull;
-            k[2] = t2 >= 0.0f ? ((uint64_t)(fbits(t2) & 0x7fffffffu) << 32) | (rf.z << 5 | cn.z) : ~0ull;
-            k[3] = t3 >= 0.0f ? ((uint64_t)(fbits(t3) & 0x7fffffffu) << 32) | (rf.w << 5 | cn.w) : ~0ull;
-        }
+        const float t0 = cn.x != WIDE_EMPTY ? slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f),
+                                                   make_float4(hx.x, hy.x, hz.x, 0.0f), t_max) : -1.0f;
+        const float t1 = cn.y != WIDE_EMPTY ? slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f),
+                                                   make_float4(hx.y, hy.y, hz.y, 0.0f), t_max) : -1.0f;
+        const float t2 = cn.z != WIDE_EMPTY ? slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f),
+                                                   make_float4(hx.z, hy.z, hz.z, 0.0f), t_max) : -1.0f;
+        const float t3 = cn.w != WIDE_EMPTY ? slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f),
+                                                   make_float4(hx.w, hy.w, hz.w, 0.0f), t_max) : -1.0f;
         if (AnyUnordered && any) {
             // Any-hit query: t_max stays put until the walk ends at the
             // first accepted hit, so the order children are visited in
             // cannot change the answer (is some triangle on the segment
             // accepted) — only how soon it is found. Enter the first hit
             // child in slot order and push the others unsorted.
+            const float tj[4] = {t0, t1, t2, t3};
+            const uint32_t rj[4] = {rf.x, rf.y, rf.z, rf.w}, cj[4] = {cn.x, cn.y, cn.z, cn.w};
             bool entered = false;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                if (k[j] == ~0ull) continue;
+                if (!(tj[j] >= 0.0f)) continue;
                 if (!entered) {
-                    first = (uint32_t)k[j] >> 5;
-                    count = (uint32_t)k[j] & 31u;
+                    first = rj[j];
+                    count = cj[j];
                     entered = true;
                 } else if (sp < WIDE_STACK) {
-                    if (sp < in_lds) lds[sp * 64u] = k[j];
-                    else priv[sp - priv_off] = k[j];
+                    const uint64_t e = ((uint64_t)(fbits(tj[j]) & 0x7fffffffu) << 32) | (rj[j] << 5 | cj[j]);
+                    if (sp < in_lds) lds[sp * 64u] = e;
+                    else priv[sp - priv_off] = e;
                     sp++;
                 }
             }
             if (entered) return true;
         } else {
+        // Sort keys: (|t| bits << 32 | child record) for hit children, all
+        // ones for misses. A key is the stack entry itself (stack_entry's
+        // short form: every wide child record fits, first < 2^27, count <=
+        // 30), so a push stores it and the entered child's record is read
+        // back without selecting among the children. Equal entry distances
+        // are ordered by record, which only decides ties between exactly
+        // equal triangle distances, as any visiting order does.
+        uint64_t k[4];
+        k[0] = t0 >= 0.0f ? ((uint64_t)(fbits(t0) & 0x7fffffffu) << 32) | (rf.x << 5 | cn.x) : ~0ull;
+        k[1] = t1 >= 0.0f ? ((uint64_t)(fbits(t1) & 0x7fffffffu) << 32) | (rf.y << 5 | cn.y) : ~0ull;
+        k[2] = t2 >= 0.0f ? ((uint64_t)(fbits(t2) & 0x7fffffffu) << 32) | (rf.z << 5 | cn.z) : ~0ull;
+        k[3] = t3 >= 0.0f ? ((uint64_t)(fbits(t3) & 0x7fffffffu) << 32) | (rf.w << 5 | cn.w) : ~0ull;
 #define OXD_CSWAP(a, b)                                 \
     {                                                   \
         const uint64_t lo_ = k[a] < k[b] ? k[a] : k[b]; \
