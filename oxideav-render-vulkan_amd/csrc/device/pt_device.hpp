@@ -603,28 +603,42 @@ __device__ __forceinline__ bool leaf_hits(const Scene &S, const Ray &r, uint32_t
 constexpr uint32_t WIDE_STACK = 96;
 constexpr uint32_t WIDE_EMPTY = 0xffffffffu;
 
-// One step of the 4-wide walk from the node or leaf (first, count): a
-// leaf's next triangle is tested, an inner node's hit children are ordered
-// and the nearest entered; when a leaf is done, nothing is entered, or
-// there is no current node (first = WIDE_EMPTY), one stack entry is popped.
-// Returns false when the walk is over (stack empty, or an any-hit query
-// answered).
+// One step of the 4-wide walk. A lane holds a current node (first, count)
+// and a pending leaf (lf, lc: its untested slots). A step tests the pending
+// leaf's next triangle and, in the same step, takes the current node: an
+// inner node's hit children are ordered and the nearest entered; a leaf
+// becomes the pending leaf once none is pending (until then the lane only
+// drains the pending one); with no current node (first = WIDE_EMPTY) one
+// stack entry is popped. So a lane walks on to its next node while it tests
+// a leaf's triangles, and the triangle and node work of a wave overlap
+// instead of alternating. Each leaf's triangles are tested in slot order and
+// every leaf entered before t_max is tested, so the closest accepted hit is
+// the ordered walk's; only the order between leaves differs, which decides
+// nothing but exact ties in t. Returns false when the walk is over (nothing
+// pending, no current node, empty stack, or an any-hit query answered).
 // AnyUnordered: any-hit queries skip the ordering (the answer is the same;
 // faster where shadow rays to punctual lights dominate, slower on open
 // scenes whose shadow rays find near occluders sooner in distance order).
 template <bool AnyUnordered = false>
 __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t_max, Hit &best, uint32_t &first,
-                                          uint32_t &count, uint32_t &sp, uint32_t path_seed, uint32_t ray_id,
-                                          bool primary, bool any, uint32_t skip, lds_u64 *lds, uint64_t *priv,
-                                          uint32_t in_lds, uint32_t priv_off) {
+                                          uint32_t &count, uint32_t &lf, uint32_t &lc, uint32_t &sp,
+                                          uint32_t path_seed, uint32_t ray_id, bool primary, bool any, uint32_t skip,
+                                          lds_u64 *lds, uint64_t *priv, uint32_t in_lds, uint32_t priv_off) {
+    // The pending leaf's next triangle (its triangles in slot order, as
+    // leaf_hits takes them, one per step).
+    if (lc > 0) {
+        if (leaf_hits(S, r, lf, 1, t_max, best, path_seed, ray_id, primary, any, skip)) return false;
+        lf++;
+        lc--;
+    }
     if (count > 0) {
-        // One triangle per step (the leaf's triangles in slot order, as
-        // leaf_hits takes them): a lane at a leaf costs its wave one
-        // triangle test per step rather than the longest leaf of the wave.
-        if (leaf_hits(S, r, first, 1, t_max, best, path_seed, ray_id, primary, any, skip)) return false;
-        first++;
-        count--;
-        if (count > 0) return true;
+        // The current node is a leaf: it becomes the pending leaf once that
+        // is drained, and the walk goes on to the next node meanwhile.
+        if (lc > 0) return true;
+        lf = first;
+        lc = count;
+        first = WIDE_EMPTY;
+        count = 0;
     } else if (first != WIDE_EMPTY) {
         const gf4_ptr nd = S.geom + S.wide_base + 8u * first;
         const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
@@ -647,6 +661,8 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
             const float tj[4] = {t0, t1, t2, t3};
             const uint32_t rj[4] = {rf.x, rf.y, rf.z, rf.w}, cj[4] = {cn.x, cn.y, cn.z, cn.w};
             bool entered = false;
+            first = WIDE_EMPTY;
+            count = 0;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 if (!(tj[j] >= 0.0f)) continue;
@@ -661,7 +677,14 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
                     sp++;
                 }
             }
-            if (entered) return true;
+            if (count > 0 && lc == 0) {
+                lf = first;
+                lc = count;
+                first = WIDE_EMPTY;
+                count = 0;
+            } else if (entered) {
+                return true;
+            }
         } else {
         // Sort keys: (|t| bits << 32 | child record) for hit children, all
         // ones for misses. A key is the stack entry itself (stack_entry's
@@ -697,14 +720,20 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
             }
             first = (uint32_t)k[0] >> 5;
             count = (uint32_t)k[0] & 31u;
-            return true;
+            if (count == 0 || lc > 0) return true;
+            // The nearest child is a leaf and none is pending: it becomes
+            // the pending leaf and the next node is popped in this step.
+            lf = first;
+            lc = count;
         }
+        first = WIDE_EMPTY;
+        count = 0;
         }
     }
     // One pop per step as well: an entry beyond t_max leaves the lane with
     // no current node (first = WIDE_EMPTY), and the next step pops again,
     // so a lane discarding entries does not hold its wave in a pop loop.
-    if (sp == 0) return false;
+    if (sp == 0) return lc > 0;
     sp--;
     const uint64_t e = sp < in_lds ? lds[sp * 64u] : priv[sp - priv_off];
     if (bitsf((uint32_t)(e >> 32)) <= t_max) {
@@ -734,8 +763,9 @@ __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_m
     uint64_t priv[LdsDepth > 0 ? WIDE_STACK - LdsDepth : WIDE_STACK];
     constexpr uint32_t priv_off = LdsDepth > 0 ? (uint32_t)LdsDepth : 0u;
     uint32_t sp = 0, first = 0, count = 0;  // wide node 0, inner
-    while (wide_step<AnyUnordered>(S, r, t_max, best, first, count, sp, path_seed, ray_id, primary, any, skip, lds,
-                                   priv, in_lds, priv_off)) {
+    uint32_t lf = 0, lc = 0;                // no pending leaf
+    while (wide_step<AnyUnordered>(S, r, t_max, best, first, count, lf, lc, sp, path_seed, ray_id, primary, any, skip,
+                                   lds, priv, in_lds, priv_off)) {
     }
     return best;
 }
