@@ -315,8 +315,13 @@ typedef struct oxpt_tuning {
                                 per-lane queue in device memory, (3 * lights + 12) * 4 bytes per
                                 resident lane); OXPT_EARLY_OFF = the general forms. Same images
                                 either way */
+    uint32_t opaque;         /* early forms: 0 = default, scenes whose materials all have alpha mode OPAQUE
+                                (every job's, in a batch) run the traversal built without the alpha test
+                                (no texture lookup in the walk loop); OXPT_OPAQUE_OFF = always the
+                                general traversal. Same images either way */
 } oxpt_tuning;
 #define OXPT_EARLY_OFF 0xFFFFu
+#define OXPT_OPAQUE_OFF 0xFFFFu
 #define OXPT_STACK_PRIVATE 0xFFFFu
 #define OXPT_TAIL_NONE 0xFFFFu
 int oxpt_ctx_set_tuning(oxpt_ctx *ctx, const oxpt_tuning *t);

@@ -435,6 +435,21 @@ __device__ __forceinline__ bool accept(const Scene &S, uint32_t slot, F3 b, bool
     return coin(path_seed, ray_id, gid) < hit_alpha(S, slot, b);
 }
 
+// Acceptance of a triangle hit that passed the skip test. Opaque walks are
+// built for scenes whose every material has alpha mode OPAQUE (flags bits
+// 0-1 clear): accept() then only culls back faces of camera rays on
+// single-sided materials, so the material word is read for those alone and
+// the alpha lookup (a texture sample) is not compiled into the walk loop.
+template <bool Opaque>
+__device__ __forceinline__ bool accept_hit(const Scene &S, uint32_t slot, F3 b, bool front, uint32_t mat,
+                                           uint32_t path_seed, uint32_t ray_id, bool primary, uint32_t gid) {
+    if constexpr (Opaque) {
+        return !(primary && !front && (mat_flags(S, mat) & 4u) == 0);
+    } else {
+        return accept(S, slot, b, front, mat, mat_flags(S, mat), path_seed, ray_id, primary, gid);
+    }
+}
+
 // Traversal stack: 64 entries (the reference's depth, wgsl:661-662), each
 // (t_far bits << 32 | node). The bottom `lds_depth` entries of every lane
 // live in LDS, laid out [entry][lane] in the wave's own slice so lanes at
@@ -464,7 +479,7 @@ __device__ __forceinline__ uint64_t stack_entry(float t_far, uint32_t first, uin
 // count) travels with it — read from the parent's child pair on descent,
 // from the stack entry on a pop — so each inner-node step issues only the
 // one 64-byte child-pair load.
-template <int LdsDepth>
+template <int LdsDepth, bool Opaque = false>
 __device__ __forceinline__ Hit traverse2(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                          uint32_t ray_id, bool primary, bool any, uint32_t skip, lds_u64 *lds_stack,
                                          uint32_t lds_depth) {
@@ -499,9 +514,7 @@ __device__ __forceinline__ Hit traverse2(const Scene &S, const Ray &r, float t_m
                 if (!intersect_tri(r, xyz(w0), xyz(w1), xyz(w2), t_max, t, b, front)) continue;
                 uint32_t gid = fbits(w0.w);
                 if (gid == skip) continue;
-                uint32_t mat = fbits(w2.w);
-                uint32_t fl = mat_flags(S, mat);
-                if (!accept(S, slot, b, front, mat, fl, path_seed, ray_id, primary, gid)) continue;
+                if (!accept_hit<Opaque>(S, slot, b, front, fbits(w2.w), path_seed, ray_id, primary, gid)) continue;
                 best.slot = slot;
                 best.t = t;
                 best.b = b;
@@ -568,6 +581,7 @@ __device__ __forceinline__ Hit traverse2(const Scene &S, const Ray &r, float t_m
 
 // Leaf: test slots [first, first + count) in order (wgsl:683-699); returns
 // true when an any-hit query is answered.
+template <bool Opaque>
 __device__ __forceinline__ bool leaf_hits(const Scene &S, const Ray &r, uint32_t first, uint32_t count, float &t_max,
                                           Hit &best, uint32_t path_seed, uint32_t ray_id, bool primary, bool any,
                                           uint32_t skip) {
@@ -579,9 +593,7 @@ __device__ __forceinline__ bool leaf_hits(const Scene &S, const Ray &r, uint32_t
         if (!intersect_tri(r, xyz(w0), xyz(w1), xyz(w2), t_max, t, b, front)) continue;
         uint32_t gid = fbits(w0.w);
         if (gid == skip) continue;
-        uint32_t mat = fbits(w2.w);
-        uint32_t fl = mat_flags(S, mat);
-        if (!accept(S, slot, b, front, mat, fl, path_seed, ray_id, primary, gid)) continue;
+        if (!accept_hit<Opaque>(S, slot, b, front, fbits(w2.w), path_seed, ray_id, primary, gid)) continue;
         best.slot = slot;
         best.t = t;
         best.b = b;
@@ -619,7 +631,7 @@ constexpr uint32_t WIDE_EMPTY = 0xffffffffu;
 // AnyUnordered: any-hit queries skip the ordering (the answer is the same;
 // faster where shadow rays to punctual lights dominate, slower on open
 // scenes whose shadow rays find near occluders sooner in distance order).
-template <bool AnyUnordered = false>
+template <bool AnyUnordered = false, bool Opaque = false>
 __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t_max, Hit &best, uint32_t &first,
                                           uint32_t &count, uint32_t &lf, uint32_t &lc, uint32_t &sp,
                                           uint32_t path_seed, uint32_t ray_id, bool primary, bool any, uint32_t skip,
@@ -627,7 +639,7 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
     // The pending leaf's next triangle (its triangles in slot order, as
     // leaf_hits takes them, one per step).
     if (lc > 0) {
-        if (leaf_hits(S, r, lf, 1, t_max, best, path_seed, ray_id, primary, any, skip)) return false;
+        if (leaf_hits<Opaque>(S, r, lf, 1, t_max, best, path_seed, ray_id, primary, any, skip)) return false;
         lf++;
         lc--;
     }
@@ -644,14 +656,14 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
         const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
         const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
         const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
-        const float t0 = cn.x != WIDE_EMPTY ? slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f),
-                                                   make_float4(hx.x, hy.x, hz.x, 0.0f), t_max) : -1.0f;
-        const float t1 = cn.y != WIDE_EMPTY ? slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f),
-                                                   make_float4(hx.y, hy.y, hz.y, 0.0f), t_max) : -1.0f;
-        const float t2 = cn.z != WIDE_EMPTY ? slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f),
-                                                   make_float4(hx.z, hy.z, hz.z, 0.0f), t_max) : -1.0f;
-        const float t3 = cn.w != WIDE_EMPTY ? slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f),
-                                                   make_float4(hx.w, hy.w, hz.w, 0.0f), t_max) : -1.0f;
+        const float s0 = slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f), make_float4(hx.x, hy.x, hz.x, 0.0f), t_max);
+        const float s1 = slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), t_max);
+        const float s2 = slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), t_max);
+        const float s3 = slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), t_max);
+        // Every child box is tested and empty slots are selected away
+        // afterwards (a select is cheaper than a branch per child).
+        const float t0 = cn.x != WIDE_EMPTY ? s0 : -1.0f, t1 = cn.y != WIDE_EMPTY ? s1 : -1.0f;
+        const float t2 = cn.z != WIDE_EMPTY ? s2 : -1.0f, t3 = cn.w != WIDE_EMPTY ? s3 : -1.0f;
         if (AnyUnordered && any) {
             // Any-hit query: t_max stays put until the walk ends at the
             // first accepted hit, so the order children are visited in
@@ -746,7 +758,7 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
     return true;
 }
 
-template <int LdsDepth, bool AnyUnordered = false>
+template <int LdsDepth, bool AnyUnordered = false, bool Opaque = false>
 __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                          uint32_t ray_id, bool primary, bool any, uint32_t skip, lds_u64 *lds_stack,
                                          uint32_t lds_depth) {
@@ -764,7 +776,7 @@ __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_m
     constexpr uint32_t priv_off = LdsDepth > 0 ? (uint32_t)LdsDepth : 0u;
     uint32_t sp = 0, first = 0, count = 0;  // wide node 0, inner
     uint32_t lf = 0, lc = 0;                // no pending leaf
-    while (wide_step<AnyUnordered>(S, r, t_max, best, first, count, lf, lc, sp, path_seed, ray_id, primary, any, skip,
+    while (wide_step<AnyUnordered, Opaque>(S, r, t_max, best, first, count, lf, lc, sp, path_seed, ray_id, primary, any, skip,
                                    lds, priv, in_lds, priv_off)) {
     }
     return best;
@@ -772,14 +784,14 @@ __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_m
 
 // Closest (or, with `any`, some) accepted hit of the ray up to t_max_in:
 // the 4-wide walk when the scene carries one, else the binary walk.
-template <int LdsDepth = 0, bool AnyUnordered = false>
+template <int LdsDepth = 0, bool AnyUnordered = false, bool Opaque = false>
 __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                         uint32_t ray_id, bool primary, bool any, uint32_t skip,
                                         lds_u64 *lds_stack = nullptr, uint32_t lds_depth = 0) {
     if (S.wide_base != 0)
-        return traverse4<LdsDepth, AnyUnordered>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack,
+        return traverse4<LdsDepth, AnyUnordered, Opaque>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack,
                                                  lds_depth);
-    return traverse2<LdsDepth>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack, lds_depth);
+    return traverse2<LdsDepth, Opaque>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack, lds_depth);
 }
 
 __device__ __forceinline__ float offset_one(float p, float n) {

@@ -90,10 +90,12 @@ struct BatchArgs {
 // nee_queue_words(max punctual) words per lane). Same images in every mode.
 constexpr uint32_t kQueueMaxPunctual = 31;
 inline uint32_t nee_queue_words(uint32_t n_punct) { return 3u * n_punct + 12u; }
+// opaque: every job's scene has only alpha-mode OPAQUE materials (the
+// early modes then run the walk built without the alpha test).
 hipError_t launch_trace_batch(const BatchArgs &args, uint32_t kernel, uint32_t block, uint32_t grid,
-                              uint32_t lds_depth, uint32_t mode, hipStream_t s);
+                              uint32_t lds_depth, uint32_t mode, bool opaque, hipStream_t s);
 hipError_t launch_trace(const TraceArgs &args, uint32_t kernel, uint32_t block, uint32_t grid, uint32_t lds_depth,
-                        uint32_t mode, hipStream_t s);
+                        uint32_t mode, bool opaque, hipStream_t s);
 hipError_t launch_resolve(const float4 *accum, uint32_t n, const oxpt_resolve_params &rp, void *out, hipStream_t s);
 hipError_t launch_debug_pattern(const uint32_t *tables, uint32_t off_sobol, uint32_t seed, uint32_t n,
                                 const uint32_t *x, const uint32_t *y, const uint32_t *index, const uint32_t *p,

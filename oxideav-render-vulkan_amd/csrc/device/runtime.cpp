@@ -73,6 +73,21 @@ uint32_t stack_lds_entries(const oxpt_tuning &t, uint32_t block, bool batched) {
     return std::min(16u, (per_wg - fixed) / ((block / 64u) * 512u));
 }
 
+// Whether the opaque walk applies to a scene under these Params: every
+// material word in the material section [offs0[2], offs0[3]) has alpha mode
+// OPAQUE (flags bits 0-1 clear). The section is read from the host copy of
+// the tables, at OXPT_MAT_STRIDE words per material; a section that is not
+// a whole number of materials, or lies outside the tables, counts as not
+// opaque (the general walk, which is right for every scene).
+bool opaque_walk(const oxpt_tuning &t, const std::vector<uint32_t> &tables, const oxpt_params &P) {
+    if (t.opaque == OXPT_OPAQUE_OFF) return false;
+    const uint64_t a = P.offs0[2], b = P.offs0[3];
+    if (b < a || b > tables.size() || (b - a) % OXPT_MAT_STRIDE != 0) return false;
+    for (uint64_t m = a; m < b; m += OXPT_MAT_STRIDE)
+        if (tables[m + OXPT_M_FLAGS] & 3u) return false;
+    return true;
+}
+
 // Device opening is serialised process-wide (src/gpu.rs:978-992).
 std::mutex g_device_init;
 
@@ -116,7 +131,7 @@ struct oxpt_ctx {
     oxpt_ctx *base = nullptr;
     std::shared_ptr<Domain> dom;
     uint32_t n_cu = 0;
-    oxpt_tuning tuning{256, 32, 1u << 30, 0, 0, 0, 0, 0, 0};
+    oxpt_tuning tuning{256, 32, 1u << 30, 0, 0, 0, 0, 0, 0, 0};
     uint32_t *d_work = nullptr;
     oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[job_cap]
     uint32_t *d_counters = nullptr;  // job_cap counters + 1 count word
@@ -125,6 +140,7 @@ struct oxpt_ctx {
     hipGraphExec_t graph = nullptr;  // captured batched step (reset + trace)
     uint32_t graph_block = 0, graph_grid = 0, graph_kernel = 0, graph_lds = 0;
     uint32_t graph_mode = 0;
+    bool graph_opaque = false;
     float *d_nee_q = nullptr;  // NEE queue of the early form with punctual lights
     size_t nee_q_bytes = 0;
     const char *last_kernel = "";  // name of the last trace kernel launched
@@ -647,12 +663,15 @@ int oxpt_ctx_set_tuning(oxpt_ctx *c, const oxpt_tuning *t) {
         return fail(c, OXPT_E_ARG, "tuning: tail must be 0 (default), 1..16 or OXPT_TAIL_NONE");
     if (t->early != 0 && t->early != OXPT_EARLY_OFF)
         return fail(c, OXPT_E_ARG, "tuning: early must be 0 (default) or OXPT_EARLY_OFF");
+    if (t->opaque != 0 && t->opaque != OXPT_OPAQUE_OFF)
+        return fail(c, OXPT_E_ARG, "tuning: opaque must be 0 (default) or OXPT_OPAQUE_OFF");
     c->tuning.kernel = t->kernel;
     c->tuning.wf_variant = t->wf_variant;
     c->tuning.stack_lds = t->stack_lds;
     c->tuning.bvh = t->bvh;
     c->tuning.tail = t->tail;
     c->tuning.early = t->early;
+    c->tuning.opaque = t->opaque;
     return OXPT_OK;
 }
 int oxpt_ctx_set_graph(oxpt_ctx *c, int enable) {
@@ -1148,6 +1167,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     // their default runs at 5 waves per SIMD (fewer live registers than the
     // general form, which keeps 4).
     const uint32_t mode = trace_mode(c->tuning, P->cfg[2]);
+    const bool opaque = mode != 0 && opaque_walk(c->tuning, s->host_tables, *P);
     oxpt_tuning form = c->tuning;
     if (form.kernel == 0 && mode == 1) form.kernel = kEarlyDefaultForm;
     if (form.kernel == 0 && mode == 2) form.kernel = kQueueDefaultForm;
@@ -1183,10 +1203,10 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         if (timed) (void)hipEventRecord(ep.a, c->stream);
         args.nee_q = c->d_nee_q;
         c->last_kernel = c->tuning.kernel == 1 ? "trace_tiles"
-                         : mode == 2           ? "trace_persistent(early, queue)"
-                         : mode == 1           ? "trace_persistent(early)"
+                         : mode == 2           ? (opaque ? "trace_persistent(early, queue, opaque)" : "trace_persistent(early, queue)")
+                         : mode == 1           ? (opaque ? "trace_persistent(early, opaque)" : "trace_persistent(early)")
                                                : "trace_persistent";
-        hipError_t e = oxd::launch_trace(args, form.kernel, block, grid, lds, mode, c->stream);
+        hipError_t e = oxd::launch_trace(args, form.kernel, block, grid, lds, mode, opaque, c->stream);
         if (e != hipSuccess) return hip_fail(c, "trace launch", e);
         if (timed) {
             (void)hipEventRecord(ep.b, c->stream);
@@ -1347,6 +1367,9 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         uint32_t max_punct = 0;
         for (const oxd::Job &J : sorted) max_punct = std::max(max_punct, J.P.cfg[2]);
         const uint32_t mode = trace_mode(c->tuning, max_punct);
+        bool opaque = mode != 0;
+        for (size_t i = 0; i < n && opaque; i++)
+            if (items[i].count > done) opaque = opaque_walk(c->tuning, items[i].scene->host_tables, items[i].params);
         oxpt_tuning form = c->tuning;
         if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = 2;
         else if (form.kernel == 0 && mode == 1) form.kernel = kEarlyDefaultForm;
@@ -1361,13 +1384,15 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         if (e != hipSuccess) return hip_fail(c, "batch job upload", e);
         oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap, c->d_samples,
                             c->d_nee_q};
-        c->last_kernel = mode == 2 ? "trace_batch(early, queue)" : mode == 1 ? "trace_batch(early)" : "trace_batch";
+        c->last_kernel = mode == 2 ? (opaque ? "trace_batch(early, queue, opaque)" : "trace_batch(early, queue)")
+                         : mode == 1 ? (opaque ? "trace_batch(early, opaque)" : "trace_batch(early)")
+                                     : "trace_batch";
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
         if (c->use_graph) {
             if (c->graph && (c->graph_block != block || c->graph_grid != grid || c->graph_kernel != form.kernel ||
-                             c->graph_lds != lds || c->graph_mode != mode)) {
+                             c->graph_lds != lds || c->graph_mode != mode || c->graph_opaque != opaque)) {
                 (void)hipGraphExecDestroy(c->graph);
                 c->graph = nullptr;
             }
@@ -1377,7 +1402,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
                 if (e == hipSuccess) {
-                    hipError_t le = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, c->stream);
+                    hipError_t le = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, opaque, c->stream);
                     e = hipStreamEndCapture(c->stream, &g);
                     if (e == hipSuccess) e = le;
                 }
@@ -1389,11 +1414,12 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
                 c->graph_kernel = form.kernel;
                 c->graph_lds = lds;
                 c->graph_mode = mode;
+                c->graph_opaque = opaque;
             }
             e = hipGraphLaunch(c->graph, c->stream);
             c->graph_launches++;
         } else {
-            e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, c->stream);
+            e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, opaque, c->stream);
         }
         if (e != hipSuccess) return hip_fail(c, "batch launch", e);
         if (timed) {
@@ -1508,6 +1534,9 @@ int oxpt_refine_batch_noalloc(oxpt_ctx *c, const oxpt_batch_item *items, size_t 
         // memory: those rounds run the general form (the same images).
         uint32_t mode = trace_mode(c->tuning, max_punct);
         if (mode == 2) mode = 0;
+        bool opaque = mode != 0;
+        for (size_t i = 0; i < n && opaque; i++)
+            if (items[i].count > done) opaque = opaque_walk(c->tuning, items[i].scene->host_tables, items[i].params);
         oxpt_tuning form = c->tuning;
         if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = 2;
         else if (form.kernel == 0 && mode == 1) form.kernel = kEarlyDefaultForm;
@@ -1517,8 +1546,8 @@ int oxpt_refine_batch_noalloc(oxpt_ctx *c, const oxpt_batch_item *items, size_t 
             e = hipMemcpyAsync(counters + kNaJobs, &nj, 4, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch (no allocation) job upload", e);
         oxd::BatchArgs args{djobs, counters + kNaJobs, counters, kNaJobs, samples, nullptr};
-        c->last_kernel = mode == 1 ? "trace_batch(early)" : "trace_batch";
-        e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, c->stream);
+        c->last_kernel = mode == 1 ? (opaque ? "trace_batch(early, opaque)" : "trace_batch(early)") : "trace_batch";
+        e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, opaque, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch (no allocation) launch", e);
         // The stack staging of the next round must not race this copy.
         e = hipStreamSynchronize(c->stream);

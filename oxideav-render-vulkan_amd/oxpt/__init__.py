@@ -144,12 +144,13 @@ class COptions(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block", u32), ("waves_per_cu", u32), ("max_paths", u32), ("kernel", u32), ("wf_variant", u32),
-                ("stack_lds", u32), ("bvh", u32), ("tail", u32), ("early", u32)]
+                ("stack_lds", u32), ("bvh", u32), ("tail", u32), ("early", u32), ("opaque", u32)]
 
 
 STACK_PRIVATE = 0xFFFF  # oxpt_tuning.stack_lds: whole traversal stack in private memory
 TAIL_NONE = 0xFFFF  # oxpt_tuning.tail: persistent kernel traces every pixel as a whole run
 EARLY_OFF = 0xFFFF  # oxpt_tuning.early: the general megakernel forms even without punctual lights
+OPAQUE_OFF = 0xFFFF  # oxpt_tuning.opaque: the general traversal even for all-opaque scenes
 
 
 class BatchItem(C.Structure):
@@ -714,8 +715,8 @@ class GpuPathTracer:
         return lib.oxpt_ctx_last_kernel(self.ctx).decode()
 
     def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0,
-                   tail=0, early=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail, early)
+                   tail=0, early=0, opaque=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail, early, opaque)
         rc = lib.oxpt_ctx_set_tuning(self.ctx, C.byref(t))
         if rc:
             _raise(rc, lib.oxpt_last_error(self.ctx).decode())
@@ -926,8 +927,8 @@ class Device:
         return n.value, ms.value
 
     def set_tuning(self, block=0, waves_per_cu=0, max_paths=0, kernel=0, wf_variant=0, stack_lds=0, bvh=0,
-                   tail=0, early=0) -> None:
-        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail, early)
+                   tail=0, early=0, opaque=0) -> None:
+        t = Tuning(block, waves_per_cu, max_paths, kernel, wf_variant, stack_lds, bvh, tail, early, opaque)
         self._check(lib.oxpt_ctx_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def get_tuning(self) -> Tuning:

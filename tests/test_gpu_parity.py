@@ -414,3 +414,44 @@ def test_early_continuation_forms_bitwise(gpu):
         out.append([a.read().view(np.uint32).copy() for a in acc])
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_opaque_walk_bitwise(gpu):
+    """Scenes whose materials all have alpha mode OPAQUE run the traversal
+    built without the alpha test: identical accumulators to the general
+    traversal (OPAQUE_OFF) for the persistent forms, the small-frame batch,
+    the NEE-queue form (opaque terrain + punctual lights) and a batch; a
+    scene with one MASK or BLEND material keeps the general traversal."""
+    env = oxpt.EnvironmentMap.sky(64, 32, 3)
+    gpu.set_environment(env)
+    opts = oxpt.RenderOptions(width=96, height=64, samples_per_pixel=4, max_bounces=8)
+    lit = small_terrain()
+    lit.add_light(kind="point", position=(0.0, 6.0, 0.0), color=(1.0, 0.9, 0.8), intensity=40.0)
+    lit.add_light(kind="directional", direction=(0.3, -1.0, 0.2), color=(0.9, 0.9, 1.0), intensity=2.0)
+    for scene, tag in ((small_terrain(), "(early"), (lit, "(early, queue")):
+        for kernel in (0, 2, 3, 4):
+            gpu.set_tuning(kernel=kernel, opaque=oxpt.OPAQUE_OFF)
+            ref = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+            assert tag in gpu.last_kernel() and "opaque" not in gpu.last_kernel(), gpu.last_kernel()
+            gpu.set_tuning(kernel=kernel)
+            got = gpu.render_hdr(scene, opts).view(np.uint32)
+            assert tag in gpu.last_kernel() and gpu.last_kernel().endswith("opaque)"), gpu.last_kernel()
+            assert np.array_equal(got, ref), (tag, kernel)
+    gpu.set_tuning(kernel=0)
+    gpu.render_hdr(ptscenes.alpha_planes(), opts)
+    assert "opaque" not in gpu.last_kernel(), gpu.last_kernel()
+    gpu.set_environment(None)
+    dev = oxpt.Device(0)
+    ds = [dev.upload(oxpt.Packed(small_terrain(), opts, env)), dev.upload(oxpt.Packed(lit, opts, env)),
+          dev.upload(oxpt.Packed(oxpt.Scene.room(1), opts))]
+    out = []
+    for opaque in (oxpt.OPAQUE_OFF, 0):
+        dev.set_tuning(opaque=opaque)
+        acc = [dev.accum(opts.width, opts.height) for _ in ds]
+        dev.refine_batch([(s, a, s.params, 0, 4) for s, a in zip(ds, acc)])
+        dev.finish()
+        assert ("opaque" in dev.last_kernel()) == (opaque == 0), dev.last_kernel()
+        out.append([a.read().view(np.uint32).copy() for a in acc])
+    dev.set_tuning()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
