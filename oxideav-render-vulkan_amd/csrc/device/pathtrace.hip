@@ -71,20 +71,22 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) tr
     A.accum[(size_t)py * w + px] = acc;
 }
 
-// Per pixel of every job (blockIdx.y = job): the launch's samples, in sample
-// order, into the accumulator. The grid is fixed (grid-stride over pixels,
-// jobs past the device-side count exit), so the step stays capturable.
+// Per tail pixel of every job (blockIdx.y = job): the launch's samples, in
+// sample order, into the accumulator (pixel runs were added by the trace
+// kernel). The grid is fixed (grid-stride over pixels, jobs past the
+// device-side count exit), so the step stays capturable.
 __global__ void __launch_bounds__(256) batch_reduce(BatchArgs B, const Job *__restrict__ jobs) {
     const uint32_t jb = blockIdx.y;
     if (jb >= min(*B.n_jobs, B.max_jobs)) return;
     const Job &J = jobs[jb];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < J.n_items; i += gridDim.x * blockDim.x) {
-        const uint32_t tile = i >> 6, in = i & 63u;
-        const uint32_t px = (tile % J.tiles_x) * 8u + (in & 7u), py = (tile / J.tiles_x) * 8u + (in >> 3);
+    const uint32_t tiles_x = J.tiles_x();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < J.n_items - J.split; i += gridDim.x * blockDim.x) {
+        const uint32_t tt = i >> 6, in = i & 63u, tile = (J.split >> 6) + tt;
+        const uint32_t px = (tile % tiles_x) * 8u + (in & 7u), py = (tile / tiles_x) * 8u + (in >> 3);
         if (px >= J.P.dims[0] || py >= J.P.dims[1]) continue;
         const size_t at = (size_t)py * J.P.dims[0] + px;
         float4 acc = J.accum[at];
-        const float4 *smp = B.samples + J.sample_base + (size_t)tile * 64u * J.count + in;
+        const float4 *smp = B.samples + J.sample_base + (size_t)tt * 64u * J.count + in;
         for (uint32_t s = 0; s < J.count; s++) {
             const float4 r = smp[(size_t)s * 64u];
             if (r.w > 0.0f) acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
@@ -98,8 +100,9 @@ __global__ void reset_counter(uint32_t *c) {
     if (threadIdx.x < 2) c[threadIdx.x] = 0;
 }
 
+// The batched launch's two counters per job (pixels, tail paths).
 __global__ void reset_counters(uint32_t *c, const uint32_t *n, uint32_t cap) {
-    uint32_t m = min(*n, cap);
+    uint32_t m = 2u * min(*n, cap);
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) c[i] = 0;
 }
 

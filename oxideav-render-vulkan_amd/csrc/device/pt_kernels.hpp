@@ -34,7 +34,12 @@ struct TraceArgs {
 
 // One job of a batched launch: a resident scene, its Params, an accumulator
 // and a sample run. 256 bytes, read with scalar loads by the wave that owns it.
-// sample_base: the job's first slot in the launch's per-path sample buffer.
+// Items [0, split) (whole 8x8 tiles) are traced as whole-pixel runs added
+// straight into the accumulator; items [split, n_items) are traced one path
+// per claim, tail_work = (n_items - split) * count paths, each path's (rgb,
+// covered) at sample_base + tile' * 64 * count + sample * 64 + in of the
+// launch's sample buffer (tile' counted from split), added in sample order
+// by batch_reduce.
 struct alignas(16) Job {
     oxpt_params P;  // 176 B
     const float4 *geom;
@@ -42,11 +47,15 @@ struct alignas(16) Job {
     const uint32_t *tables;
     const uint16_t *atlas;
     float4 *accum;
-    uint32_t atlas_side, first, count, tiles_x, n_items;
-    uint32_t sample_base;
+    uint32_t atlas_side, first, count, n_items;
+    uint32_t split, tail_work, sample_base;
     // The claim step's divisors as multiply-shift pairs (udiv_magic): the
-    // paths per tile (64 * count) and tiles_x.
-    uint32_t tile_m, tile_l, tx_m, tx_l;
+    // paths per tile (64 * count: tile_m, shift in bits 0-7 of `shifts`)
+    // and the tiles per row (tx_m, shift in bits 8-15).
+    uint32_t tile_m, tx_m, shifts;
+    __host__ __device__ uint32_t tiles_x() const { return (P.dims[0] + 7u) >> 3; }
+    __host__ __device__ uint32_t tile_l() const { return shifts & 0xffu; }
+    __host__ __device__ uint32_t tx_l() const { return (shifts >> 8) & 0xffu; }
 };
 static_assert(sizeof(Job) == 256, "Job is 256 bytes");
 
@@ -76,9 +85,9 @@ __device__ __forceinline__ uint32_t udiv(uint32_t n, uint32_t m, uint32_t l) {
 struct BatchArgs {
     const Job *jobs;
     const uint32_t *n_jobs;
-    uint32_t *counters;  // one work counter per job
-    uint32_t max_jobs;   // capacity of jobs / counters
-    float4 *samples;     // one (rgb, covered) per traced path of the launch
+    uint32_t *counters;  // two work counters per job: [2j] pixels, [2j + 1] tail paths
+    uint32_t max_jobs;   // capacity of jobs (counters: 2 * max_jobs)
+    float4 *samples;     // one (rgb, covered) per tail path of the launch
     float *nee_q;        // NEE queue of the early form with punctual lights (as TraceArgs)
 };
 

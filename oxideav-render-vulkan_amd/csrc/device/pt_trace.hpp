@@ -711,15 +711,20 @@ trace_persistent(TraceArgs A, uint32_t lds_depth) {
 }
 
 // Batched form: many (scene, camera, sample run) jobs in one persistent
-// launch. The work item is one path — (job, 8x8 tile, sample, pixel), so a
-// wave's 64 lanes take one tile at one sample index — and a lane claims
-// its next item as soon as its path ends. A wave works on one job at a time
-// (so the job's Params and scene pointers stay in scalar registers), and
-// when the job runs dry it moves on after at most one path per lane, not a
-// whole run of samples per pixel. Each path writes (rgb, covered) to the
-// launch's sample buffer; batch_reduce then adds every pixel's samples into
-// its accumulator in sample order, so every job's image equals its image
-// alone. Jobs are ordered largest first by the host.
+// launch, each job worked as trace_persistent works a frame. A wave works
+// on one job at a time (so the job's Params and scene pointers stay in
+// scalar registers). Its lanes claim whole pixels of the job first — items
+// [0, split) — and trace each pixel's samples in order into the pixel's
+// accumulator; once the job's pixels are spent, lanes claim single paths of
+// the job's tail items [split, n_items) — (tile, sample, pixel), so lanes
+// that claim together take one tile at one sample — whose (rgb, covered)
+// go to the launch's sample buffer, and batch_reduce adds them into the
+// accumulator in sample order. So lanes out of pixels take tail paths
+// instead of idling while the rest of their wave finishes whole runs, the
+// wave leaves a job after at most one tail path per lane, and every job's
+// image equals its image alone. The host orders jobs largest first and
+// gives the last ones (whose pixel runs would outlast the launch) no pixel
+// runs at all.
 template <int MinWaves, int Mode = 0, bool Opaque = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves)))
 trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
@@ -733,8 +738,6 @@ trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
     stage_tables(jobs[0].tables, jobs[0].P, lds_sobol, lds_sheen);
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
-    bool has_path = false;
-    uint32_t item = 0;
     PathState st;
     const NeeQueue Q{B.nee_q, gridDim.x * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     uint32_t j = 0;
@@ -742,71 +745,114 @@ trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
         j = __builtin_amdgcn_readfirstlane(j);
         const Job &J = jobs[j];
         const Scene S = make_scene(J.P, J.geom, J.attrs, J.tables, J.atlas, J.atlas_side, lds_sheen);
-        const uint32_t w = J.P.dims[0], h = J.P.dims[1];
-        const uint32_t per_tile = 64u * J.count, n_paths = J.n_items * J.count;
-        bool job_live = true;  // wave-uniform: job j may still have unclaimed paths
+        const uint32_t w = J.P.dims[0], h = J.P.dims[1], tiles_x = J.tiles_x();
+        uint32_t *const work = B.counters + 2u * j;  // [0] pixels, [1] tail paths
+        // A lane holds samples [done, run_end) of pixel (px, py): a pixel's
+        // whole run added into `acc`, or (to_buffer) one tail path whose
+        // sample goes to the sample buffer at slot + 64 * sample.
+        bool has_pixel = false, exhausted = false, need_init = false, to_buffer = false;
+        uint32_t px = 0, py = 0, done = 0, run_end = 0, slot = 0;
+        // Wave-uniform: the job's pixel queue is spent, claims go to its tail.
+        bool pixels_spent = J.split == 0;
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         for (;;) {
-            bool want = !has_path && job_live;
-            uint64_t m = __ballot(want);
+            const bool want = !has_pixel && !exhausted;
+            const uint64_t m = __ballot(want);
             if (m) {
-                uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+                const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+                const uint32_t n = (uint32_t)__popcll(m);
                 uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(B.counters + j, (uint32_t)__popcll(m));
-                base = __shfl(base, leader);
-                bool dry = false;
-                if (want) {
-                    const uint32_t it = base + (uint32_t)__popcll(m & lt_mask);
-                    if (it >= n_paths) {
-                        dry = true;
+                if (lane == leader) base = atomicAdd(work + (pixels_spent ? 1u : 0u), n);
+                base = __builtin_amdgcn_readfirstlane(__shfl(base, leader));
+                const uint32_t rank = (uint32_t)__popcll(m & lt_mask);
+                if (!pixels_spent) {
+                    if (want) {
+                        const uint32_t item = base + rank;
+                        // Items outside the image (ragged edge tiles) have no
+                        // pixel: the lane claims again on the next ballot.
+                        if (item < J.split) {
+                            const uint32_t tile = item >> 6, in = item & 63u;
+                            const uint32_t ty = udiv(tile, J.tx_m, J.tx_l()), tx = tile - ty * tiles_x;
+                            px = tx * 8u + (in & 7u);
+                            py = ty * 8u + (in >> 3);
+                            if (px < w && py < h) {
+                                has_pixel = true;
+                                to_buffer = false;
+                                done = 0;
+                                run_end = J.count;
+                                need_init = true;
+                                acc = J.accum[(size_t)py * w + px];
+                            }
+                        }
+                    }
+                    if (base + n >= J.split) pixels_spent = true;
+                } else if (want) {
+                    const uint32_t q = base + rank;
+                    if (q >= J.tail_work) {
+                        exhausted = true;
                     } else {
-                        const uint32_t tile = udiv(it, J.tile_m, J.tile_l), rem = it - tile * per_tile;
-                        const uint32_t s = rem >> 6, in = rem & 63u;
-                        const uint32_t ty = udiv(tile, J.tx_m, J.tx_l), tx = tile - ty * J.tiles_x;
-                        const uint32_t px = tx * 8u + (in & 7u);
-                        const uint32_t py = ty * 8u + (in >> 3);
-                        // Items outside the image (ragged edge tiles) have
-                        // no path; the reduce never reads them.
+                        const uint32_t per_tile = 64u * J.count;
+                        const uint32_t tt = udiv(q, J.tile_m, J.tile_l()), rem = q - tt * per_tile;
+                        const uint32_t in = rem & 63u, tile = (J.split >> 6) + tt;
+                        const uint32_t ty = udiv(tile, J.tx_m, J.tx_l()), tx = tile - ty * tiles_x;
+                        px = tx * 8u + (in & 7u);
+                        py = ty * 8u + (in >> 3);
                         if (px < w && py < h) {
-                            has_path = true;
-                            item = it;
-                            init_path(J.P, lds_sobol, px, py, J.first + s, st);
+                            has_pixel = true;
+                            to_buffer = true;
+                            done = rem >> 6;
+                            run_end = done + 1u;
+                            slot = J.sample_base + tt * per_tile + in;
+                            need_init = true;
                         }
                     }
                 }
-                if (__ballot(dry)) job_live = false;
             }
-            if (__ballot(has_path) == 0) {
-                if (job_live) continue;  // ragged edge items only: claim again
-                break;                   // job j is drained for this wave
+            if (__ballot(has_pixel || !exhausted) == 0) break;  // job j is drained for this wave
+            if (!has_pixel) continue;
+            if (need_init) {
+                init_path(J.P, lds_sobol, px, py, J.first + done, st);
+                need_init = false;
             }
-            if (!has_path) continue;
             // A hit's shadow rays are traced before the wave returns to the
             // claim step, so its lanes shade their next hits together: with
             // punctual lights (many shadow rays per hit) and in the early
             // form (lanes whose hit made no shadow ray would otherwise shade
-            // a step ahead of the rest; with one path per work item this took
-            // a 256x256 room at 16 spp from 669 to 857 Mpaths/s and a batch
-            // of 64 uneven terrain scenes from 747 to 803).
+            // a step ahead of the rest; on the uneven batch of 64 terrain
+            // scenes 874 -> 952 Mpaths/s, on a 256x256 room at 16 spp 688 ->
+            // 905).
             bool covered, ended;
-            while (!(ended = step_path<Mode, Opaque>(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth, Q)) && st.shadow &&
-                   (Mode == 1 || J.P.cfg[2] != 0)) {
+            while (!(ended = step_path<Mode, Opaque>(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth, Q)) &&
+                   st.shadow && (Mode == 1 || J.P.cfg[2] != 0)) {
             }
             if (ended) {
-                F3 r = covered && finite3(st.radiance) ? st.radiance : f3(0.0f);
-                B.samples[J.sample_base + item] = make_float4(r.x, r.y, r.z, covered ? 1.0f : 0.0f);
-                has_path = false;
+                if (to_buffer) {
+                    const F3 r = covered && finite3(st.radiance) ? st.radiance : f3(0.0f);
+                    B.samples[slot + 64u * done] = make_float4(r.x, r.y, r.z, covered ? 1.0f : 0.0f);
+                } else if (covered) {
+                    const F3 r = finite3(st.radiance) ? st.radiance : f3(0.0f);
+                    acc = make_float4(acc.x + r.x, acc.y + r.y, acc.z + r.z, acc.w + 1.0f);
+                }
+                done += 1;
+                if (done == run_end) {
+                    if (!to_buffer) J.accum[(size_t)py * w + px] = acc;
+                    has_pixel = false;
+                } else {
+                    need_init = true;
+                }
             }
         }
         // Next job with unclaimed work.
         uint32_t nj = j + 1;
         while (nj < n_jobs &&
-               __hip_atomic_load(B.counters + nj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= jobs[nj].n_items * jobs[nj].count)
+               __hip_atomic_load(B.counters + 2u * nj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= jobs[nj].split &&
+               __hip_atomic_load(B.counters + 2u * nj + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+                   jobs[nj].tail_work)
             nj++;
         if (nj >= n_jobs) break;
         j = nj;
     }
 }
-
 
 }  // namespace oxd
 

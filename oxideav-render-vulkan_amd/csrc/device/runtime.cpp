@@ -133,8 +133,8 @@ struct oxpt_ctx {
     uint32_t n_cu = 0;
     oxpt_tuning tuning{256, 32, 1u << 30, 0, 0, 0, 0, 0, 0, 0};
     uint32_t *d_work = nullptr;
-    oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[job_cap]
-    uint32_t *d_counters = nullptr;  // job_cap counters + 1 count word
+    oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[2 * job_cap]
+    uint32_t *d_counters = nullptr;  // two counters per job (pixels, tail paths) + 1 count word
     size_t job_cap = 0;
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;  // captured batched step (reset + trace)
@@ -620,7 +620,7 @@ void oxpt_ctx_destroy(oxpt_ctx *c) {
             if (e) (void)hipEventDestroy(e);
     }
     dfree(c, c->d_jobs, c->job_cap * sizeof(oxd::Job));
-    dfree(c, c->d_counters, c->job_cap * 4 + 4);
+    dfree(c, c->d_counters, c->job_cap * 8 + 4);
     dfree(c, c->d_samples, c->samples_bytes);
     dfree(c, c->d_nee_q, c->nee_q_bytes);
     dfree(c, c->d_out, c->d_out_bytes);
@@ -1094,6 +1094,67 @@ static int ensure_queue(oxpt_ctx *c, uint64_t lanes, uint32_t n_punct) {
     return OXPT_OK;
 }
 
+// Work split of one batched round whose jobs are sorted largest first:
+// each job's first tiles are traced as whole-pixel runs and its last
+// tail16/16 of tiles (rounded up) one path per claim, as the persistent
+// megakernel traces a frame. A job keeps pixel runs only when its tail has
+// at least one item per two resident lanes: runs end at times spread over
+// one run's length after the last pixel is claimed, so the lanes done first
+// need about half a run's worth of tail paths each. Smaller jobs are traced
+// path by path throughout (split = 0), jobs of one sample as one-sample
+// runs. Places each job's tail samples in the sample buffer and returns the
+// slots the round needs.
+static uint64_t split_round(oxd::Job *jobs, size_t nj, uint32_t tail16, uint64_t lanes) {
+    uint64_t slots = 0;
+    for (size_t k = 0; k < nj; k++) {
+        oxd::Job &J = jobs[k];
+        const uint32_t tiles = J.n_items / 64u;
+        uint32_t tail_tiles = tiles;
+        if (J.count == 1 || tail16 == 0) {
+            tail_tiles = 0;  // one-sample runs, or every pixel a run (OXPT_TAIL_NONE)
+        } else if (tail16 < 16) {
+            const uint32_t t = (uint32_t)(((uint64_t)tiles * tail16 + 15u) / 16u);
+            if ((uint64_t)t * 128u >= lanes) tail_tiles = t;
+        }
+        J.split = (tiles - tail_tiles) * 64u;
+        J.tail_work = tail_tiles * 64u * J.count;
+        J.sample_base = (uint32_t)slots;
+        slots += J.tail_work;
+    }
+    return slots;
+}
+
+// Lanes resident at once in a batched launch of the context's form.
+static uint64_t resident_lanes(const oxpt_ctx *c) {
+    const uint32_t k = c->tuning.kernel == 0 ? kEarlyDefaultForm : c->tuning.kernel;
+    return (uint64_t)c->n_cu * 4u * 64u * form_waves(k);
+}
+
+// Job fields of one batched item for a round adding `count` samples from
+// `first` (the split is set by split_round).
+static oxd::Job make_job(const oxpt_ctx *c, const oxpt_batch_item &it, uint32_t first, uint32_t count) {
+    oxd::Job J;
+    std::memset(&J, 0, sizeof J);
+    J.P = it.params;
+    if (c->tuning.bvh == 2) J.P.offs2[3] = 0;
+    J.geom = it.scene->geom;
+    J.attrs = it.scene->attrs;
+    J.tables = it.scene->tables;
+    J.atlas = it.scene->atlas;
+    J.accum = it.accum->d;
+    J.atlas_side = it.scene->atlas_side;
+    J.first = first;
+    J.count = count;
+    const uint32_t tx = (it.accum->w + 7u) / 8u, ty = (it.accum->h + 7u) / 8u;
+    J.n_items = 64u * tx * ty;
+    J.split = J.n_items;
+    const oxd::UDiv dt = oxd::udiv_magic(64u * count), dx = oxd::udiv_magic(tx);
+    J.tile_m = dt.m;
+    J.tx_m = dx.m;
+    J.shifts = dt.l | dx.l << 8;
+    return J;
+}
+
 static int ensure_samples(oxpt_ctx *c, uint64_t slots) {
     if (c->samples_bytes >= slots * 16) return OXPT_OK;
     (void)hipStreamSynchronize(c->stream);
@@ -1144,8 +1205,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
         J.atlas_side = s->atlas_side;
         J.first = first;
         J.count = count;
-        J.tiles_x = (a->w + 7u) / 8u;
-        J.n_items = 64u * J.tiles_x * ((a->h + 7u) / 8u);
+        J.n_items = 64u * J.tiles_x() * ((a->h + 7u) / 8u);
         EventPair ep{};
         bool timed = hipEventCreate(&ep.a) == hipSuccess && hipEventCreate(&ep.b) == hipSuccess;
         if (timed) (void)hipEventRecord(ep.a, c->stream);
@@ -1246,7 +1306,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
             c->graph = nullptr;
         }
         dfree(c, c->d_jobs, c->job_cap * sizeof(oxd::Job));
-        dfree(c, c->d_counters, c->job_cap * 4 + 4);
+        dfree(c, c->d_counters, c->job_cap * 8 + 4);
         c->d_jobs = nullptr;
         c->d_counters = nullptr;
         c->job_cap = 0;
@@ -1254,7 +1314,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         void *p = nullptr;
         if (int rc = dalloc(c, cap * sizeof(oxd::Job), &p, "batch job table")) return rc;
         c->d_jobs = (oxd::Job *)p;
-        if (int rc = dalloc(c, cap * 4 + 4, &p, "batch counters")) {
+        if (int rc = dalloc(c, cap * 8 + 4, &p, "batch counters")) {
             dfree(c, c->d_jobs, cap * sizeof(oxd::Job));
             c->d_jobs = nullptr;
             return rc;
@@ -1279,8 +1339,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
             J.atlas_side = it.scene->atlas_side;
             J.first = it.first;
             J.count = it.count;
-            J.tiles_x = (it.accum->w + 7u) / 8u;
-            J.n_items = 64u * J.tiles_x * ((it.accum->h + 7u) / 8u);
+            J.n_items = 64u * J.tiles_x() * ((it.accum->h + 7u) / 8u);
             jobs.push_back(J);
         }
         if (jobs.empty()) return OXPT_OK;
@@ -1307,57 +1366,35 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
     uint32_t spd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_count, c->tuning.max_paths / std::max<uint64_t>(total, 1)));
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
-    // One sample slot per traced path of a round (8x8 tiles incl. ragged
-    // edges x samples of the round), sized for the largest round.
-    uint64_t slots = 0;
-    for (size_t i = 0; i < n; i++) {
-        const uint64_t tiles = (uint64_t)((items[i].accum->w + 7u) / 8u) * ((items[i].accum->h + 7u) / 8u);
-        slots += 64u * tiles * std::min(spd, items[i].count);
-    }
-    if (slots >= (1ull << 32)) return fail(c, OXPT_E_ARG, "batch: round too large (lower max_paths)");
-    if (int rc = ensure_samples(c, slots)) return rc;
-    std::vector<oxd::Job> jobs;
+    const uint32_t tail16 = c->tuning.tail == 0 ? kDefaultTail : (c->tuning.tail == OXPT_TAIL_NONE ? 0u : c->tuning.tail);
+    // The jobs of round `done` (items with samples left), largest first,
+    // split into pixel runs and path tails; returns the round's sample slots.
+    std::vector<oxd::Job> sorted;
     std::vector<double> work;
-    for (uint32_t done = 0; done < max_count; done += spd) {
-        jobs.clear();
+    auto build_round = [&](uint32_t done) -> uint64_t {
+        std::vector<oxd::Job> jobs;
         work.clear();
         for (size_t i = 0; i < n; i++) {
             const oxpt_batch_item &it = items[i];
             if (it.count <= done) continue;
-            oxd::Job J;
-            std::memset(&J, 0, sizeof J);
-            J.P = it.params;
-            if (c->tuning.bvh == 2) J.P.offs2[3] = 0;
-            J.geom = it.scene->geom;
-            J.attrs = it.scene->attrs;
-            J.tables = it.scene->tables;
-            J.atlas = it.scene->atlas;
-            J.accum = it.accum->d;
-            J.atlas_side = it.scene->atlas_side;
-            J.first = it.first + done;
-            J.count = std::min(spd, it.count - done);
-            uint32_t tx = (it.accum->w + 7u) / 8u, ty = (it.accum->h + 7u) / 8u;
-            J.tiles_x = tx;
-            J.n_items = 64u * tx * ty;
-            const oxd::UDiv dt = oxd::udiv_magic(64u * J.count), dx = oxd::udiv_magic(tx);
-            J.tile_m = dt.m;
-            J.tile_l = dt.l;
-            J.tx_m = dx.m;
-            J.tx_l = dx.l;
-            jobs.push_back(J);
-            work.push_back((double)it.accum->w * it.accum->h * J.count);
+            jobs.push_back(make_job(c, it, it.first + done, std::min(spd, it.count - done)));
+            work.push_back((double)it.accum->w * it.accum->h * jobs.back().count);
         }
-        // Largest job first: the small ones fill the tail.
+        // Largest job first: the small ones fill the end of the launch.
         std::vector<size_t> order(jobs.size());
         for (size_t k = 0; k < order.size(); k++) order[k] = k;
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
-        std::vector<oxd::Job> sorted(jobs.size());
-        uint32_t sample_base = 0;
-        for (size_t k = 0; k < order.size(); k++) {
-            sorted[k] = jobs[order[k]];
-            sorted[k].sample_base = sample_base;
-            sample_base += sorted[k].n_items * sorted[k].count;
-        }
+        sorted.resize(jobs.size());
+        for (size_t k = 0; k < order.size(); k++) sorted[k] = jobs[order[k]];
+        return split_round(sorted.data(), sorted.size(), tail16, resident_lanes(c));
+    };
+    // One sample slot per tail path, sized for the largest round.
+    uint64_t slots = 0;
+    for (uint32_t done = 0; done < max_count; done += spd) slots = std::max(slots, build_round(done));
+    if (slots >= (1ull << 32)) return fail(c, OXPT_E_ARG, "batch: round too large (lower max_paths)");
+    if (int rc = ensure_samples(c, std::max<uint64_t>(slots, 1))) return rc;
+    for (uint32_t done = 0; done < max_count; done += spd) {
+        build_round(done);
         uint32_t nj = (uint32_t)sorted.size();
         // Default form: 4 waves per SIMD, but 3 for a small round (a few
         // paths per resident lane), where the end of the round — one path's
@@ -1380,9 +1417,9 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         hipError_t e = hipMemcpyAsync(c->d_jobs, sorted.data(), sorted.size() * sizeof(oxd::Job), hipMemcpyHostToDevice,
                                       c->stream);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(c->d_counters + c->job_cap, &nj, 4, hipMemcpyHostToDevice, c->stream);
+            e = hipMemcpyAsync(c->d_counters + 2 * c->job_cap, &nj, 4, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch job upload", e);
-        oxd::BatchArgs args{c->d_jobs, c->d_counters + c->job_cap, c->d_counters, (uint32_t)c->job_cap, c->d_samples,
+        oxd::BatchArgs args{c->d_jobs, c->d_counters + 2 * c->job_cap, c->d_counters, (uint32_t)c->job_cap, c->d_samples,
                             c->d_nee_q};
         c->last_kernel = mode == 2 ? (opaque ? "trace_batch(early, queue, opaque)" : "trace_batch(early, queue)")
                          : mode == 1 ? (opaque ? "trace_batch(early, opaque)" : "trace_batch(early)")
@@ -1480,6 +1517,7 @@ int oxpt_refine_batch_noalloc(oxpt_ctx *c, const oxpt_batch_item *items, size_t 
     const uint32_t spd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_count, cap / pix64));
     const uint32_t block = c->tuning.block;
     const uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
+    const uint32_t tail16 = c->tuning.tail == 0 ? kDefaultTail : (c->tuning.tail == OXPT_TAIL_NONE ? 0u : c->tuning.tail);
     // Host staging on the stack (about 67 KiB): no heap allocation either.
     oxd::Job jobs[kNaJobs];
     double work[kNaJobs];
@@ -1503,33 +1541,14 @@ int oxpt_refine_batch_noalloc(oxpt_ctx *c, const oxpt_batch_item *items, size_t 
             order[k] = (uint32_t)i;
             round_paths += wk;
         }
-        uint32_t max_punct = 0, sample_base = 0;
+        uint32_t max_punct = 0;
         for (uint32_t k = 0; k < nj; k++) {
             const oxpt_batch_item &it = items[order[k]];
-            oxd::Job &J = jobs[k];
-            std::memset(&J, 0, sizeof J);
-            J.P = it.params;
-            if (c->tuning.bvh == 2) J.P.offs2[3] = 0;
-            J.geom = it.scene->geom;
-            J.attrs = it.scene->attrs;
-            J.tables = it.scene->tables;
-            J.atlas = it.scene->atlas;
-            J.accum = it.accum->d;
-            J.atlas_side = it.scene->atlas_side;
-            J.first = it.first + done;
-            J.count = std::min(spd, it.count - done);
-            const uint32_t tx = (it.accum->w + 7u) / 8u, ty = (it.accum->h + 7u) / 8u;
-            J.tiles_x = tx;
-            J.n_items = 64u * tx * ty;
-            const oxd::UDiv dt = oxd::udiv_magic(64u * J.count), dx = oxd::udiv_magic(tx);
-            J.tile_m = dt.m;
-            J.tile_l = dt.l;
-            J.tx_m = dx.m;
-            J.tx_l = dx.l;
-            J.sample_base = sample_base;
-            sample_base += J.n_items * J.count;
-            max_punct = std::max(max_punct, J.P.cfg[2]);
+            jobs[k] = make_job(c, it, it.first + done, std::min(spd, it.count - done));
+            max_punct = std::max(max_punct, it.params.cfg[2]);
         }
+        // Tail paths are at most every path of the round: within the scratch.
+        split_round(jobs, nj, tail16, resident_lanes(c));
         // The NEE queue of the early form with punctual lights would need
         // memory: those rounds run the general form (the same images).
         uint32_t mode = trace_mode(c->tuning, max_punct);
@@ -1543,9 +1562,9 @@ int oxpt_refine_batch_noalloc(oxpt_ctx *c, const oxpt_batch_item *items, size_t 
         const uint32_t lds = stack_lds_entries(form, block, true);
         hipError_t e = hipMemcpyAsync(djobs, jobs, nj * sizeof(oxd::Job), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(counters + kNaJobs, &nj, 4, hipMemcpyHostToDevice, c->stream);
+            e = hipMemcpyAsync(counters + 2 * kNaJobs, &nj, 4, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch (no allocation) job upload", e);
-        oxd::BatchArgs args{djobs, counters + kNaJobs, counters, kNaJobs, samples, nullptr};
+        oxd::BatchArgs args{djobs, counters + 2 * kNaJobs, counters, kNaJobs, samples, nullptr};
         c->last_kernel = mode == 1 ? (opaque ? "trace_batch(early, opaque)" : "trace_batch(early)") : "trace_batch";
         e = oxd::launch_trace_batch(args, form.kernel, block, grid, lds, mode, opaque, c->stream);
         if (e != hipSuccess) return hip_fail(c, "batch (no allocation) launch", e);

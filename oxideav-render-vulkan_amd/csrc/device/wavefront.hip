@@ -81,8 +81,8 @@ __device__ __forceinline__ bool decode_item(const WfArgs &A, uint32_t item, uint
     uint32_t tile = local / per_tile, rem = local - tile * per_tile;
     s = rem >> 6;
     uint32_t in = rem & 63u;
-    px = (tile % J.tiles_x) * 8u + (in & 7u);
-    py = (tile / J.tiles_x) * 8u + (in >> 3);
+    px = (tile % J.tiles_x()) * 8u + (in & 7u);
+    py = (tile / J.tiles_x()) * 8u + (in >> 3);
     return px < J.P.dims[0] && py < J.P.dims[1];
 }
 
@@ -496,7 +496,7 @@ __global__ void __launch_bounds__(256) wf_reduce(WfArgs A, uint32_t job) {
     const uint32_t n_pix = J.n_items;  // 64 * tiles
     if (i >= n_pix) return;
     uint32_t tile = i >> 6, in = i & 63u;
-    uint32_t px = (tile % J.tiles_x) * 8u + (in & 7u), py = (tile / J.tiles_x) * 8u + (in >> 3);
+    uint32_t px = (tile % J.tiles_x()) * 8u + (in & 7u), py = (tile / J.tiles_x()) * 8u + (in >> 3);
     if (px >= J.P.dims[0] || py >= J.P.dims[1]) return;
     size_t at = (size_t)py * J.P.dims[0] + px;
     float4 acc = J.accum[at];

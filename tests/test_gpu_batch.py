@@ -89,3 +89,27 @@ def test_batch_forms_bitwise():
             assert np.array_equal(a.read().view(np.uint32), alone[i].view(np.uint32)), (kernel, bvh, graph, i)
     dev.set_graph(False)
     dev.set_tuning(kernel=0)
+
+
+def test_batch_pixel_runs_and_tails_equal_alone():
+    """A batched job is traced as whole-pixel runs (added straight into the
+    accumulator) and a path-by-path tail (added by the reduce in sample
+    order). Every split gives each job's image alone, bit for bit: all
+    pixel runs (TAIL_NONE), the default (small jobs path by path), quarter
+    and whole-frame tails, and with rounds of a few samples."""
+    dev, jobs, ds, acc = _setup(n=10, scale=0.02)
+    items = [(s, a, s.params, 0, j.opts.samples_per_pixel) for j, s, a in zip(jobs, ds, acc)]
+    alone = []
+    for j, s, a in zip(jobs, ds, acc):
+        a.clear()
+        dev.refine(s, a, s.params, 0, j.opts.samples_per_pixel)
+        alone.append(a.read().view(np.uint32).copy())
+    for tail, max_paths in ((oxpt.TAIL_NONE, 1 << 30), (0, 1 << 30), (4, 1 << 30), (16, 1 << 30),
+                            (oxpt.TAIL_NONE, 60000), (4, 60000)):
+        dev.set_tuning(tail=tail, max_paths=max_paths)
+        for a in acc:
+            a.clear()
+        dev.refine_batch(items)
+        for i, a in enumerate(acc):
+            assert np.array_equal(a.read().view(np.uint32), alone[i]), (tail, max_paths, i)
+    dev.set_tuning(max_paths=1 << 30)
