@@ -67,9 +67,9 @@ def workload(name):
 
 # Committed PMC readings per workload: file and the default form they were taken on.
 PMC_FILES = {
-    "c1": ("trace_persistent_c1_pmc.json", "trace_persistent<5,early>"),  # no punctual lights
-    "c2": ("trace_persistent_c2_pmc.json", "trace_persistent<4,queue>"),  # 16 punctual lights
-    "c0": ("trace_batch_c0_pmc.json", "trace_batch<3,early>"),  # the one-job batch of a small frame
+    "c1": ("trace_persistent_c1_pmc.json", "trace_persistent<5,early,opaque>"),  # no punctual lights
+    "c2": ("trace_persistent_c2_pmc.json", "trace_persistent<4,queue>"),  # 16 punctual lights, alpha blending
+    "c0": ("trace_batch_c0_pmc.json", "trace_batch<3,early,opaque>"),  # the one-job batch of a small frame
 }
 
 
@@ -167,7 +167,7 @@ def bench_uneven_batch(args):
 
     def per_request(kernel):
         def run():
-            dev.set_tuning(kernel=kernel, max_paths=1 << 30)
+            dev.set_tuning(kernel=kernel, max_paths=0xFFFFFFFF)
             done = []
             t0 = time.perf_counter()
             for j, s, a in zip(jobs, ds, acc):
@@ -179,7 +179,7 @@ def bench_uneven_batch(args):
 
     tuned = None
     if args.kernel == 0:
-        dev.set_tuning(kernel=0, max_paths=1 << 30)
+        dev.set_tuning(kernel=0, max_paths=0xFFFFFFFF)
         tuned, tune_ms = dev.autotune_batch(items, spp_per_trial=16)
         log(f"autotune_batch: kernel {tuned.kernel}, max_paths {tuned.max_paths}, stack_lds {tuned.stack_lds} "
             f"({tune_ms:.1f} ms per 16-spp trial)")
@@ -189,7 +189,7 @@ def bench_uneven_batch(args):
             if tuned is not None:
                 dev.set_tuning(kernel=tuned.kernel, max_paths=tuned.max_paths, stack_lds=tuned.stack_lds)
             else:
-                dev.set_tuning(kernel=args.kernel, max_paths=1 << 30)
+                dev.set_tuning(kernel=args.kernel, max_paths=0xFFFFFFFF)
             dev.set_graph(graph)
             t0 = time.perf_counter()
             dev.refine_batch(items)

@@ -289,7 +289,7 @@ uint64_t oxpt_ctx_graph_launches(const oxpt_ctx *ctx);
 typedef struct oxpt_tuning {
     uint32_t block;          /* threads per workgroup: 64, 128, 256 */
     uint32_t waves_per_cu;   /* persistent residency: workgroups = CUs * this / waves per WG */
-    uint32_t max_paths;      /* paths per launch cap (samples are split, order kept) */
+    uint32_t max_paths;      /* paths per launch cap (samples are split, order kept); default 2^32 - 1 */
     uint32_t kernel;         /* 0 = default: oxpt_refine runs the persistent megakernel at 4 waves per SIMD,
                                 5 for scenes without punctual lights (the early-continuation build; frames
                                 with fewer pixels than resident lanes: the one-job batched kernel),
@@ -335,8 +335,8 @@ int oxpt_autotune(oxpt_ctx *ctx, const oxpt_scene *scene, const oxpt_params *par
                   oxpt_tuning *best, double *best_ms);
 /* The batched launch's autotuner: times the caller's items (at most
  * spp_per_trial samples each, into scratch accumulators) under the
- * megakernel occupancy forms, samples per dispatch (max_paths 2^30, 2^28,
- * 2^26: how many rounds the batch splits into) and traversal-stack LDS
+ * megakernel occupancy forms, samples per dispatch (max_paths 2^32 - 1,
+ * 2^28, 2^26: how many rounds the batch splits into) and traversal-stack LDS
  * shares, and keeps the fastest in ctx's tuning. Same images for every
  * candidate. No counterpart in the reference (its dispatch sizes,
  * PATHS_PER_DISPATCH / PATHS_PER_SUBMIT at src/pathtrace.rs:85-95, are

@@ -27,6 +27,14 @@ constexpr uint32_t kCdfLds = 4352;
 // Persistent kernel tail in 1/16 of the frame (oxpt_tuning.tail), and the
 // most tail samples one launch buffers (16 B each: 4 GiB).
 constexpr uint32_t kDefaultTail = 8;
+// Default path cap per launch (oxpt_tuning.max_paths): as many as 32-bit
+// work indices allow, so a batch splits into as few rounds as it can — the
+// fewer the rounds, the longer each pixel run (64 uneven terrain scenes:
+// 6 rounds at 2^30 paths 952 Mpaths/s, 2 rounds 976).
+constexpr uint32_t kDefaultMaxPaths = 0xFFFFFFFFu;
+// Batched rounds hold one 16-byte sample slot per tail path; a round's
+// samples per item are halved until its slots fit this cap (16 GiB).
+constexpr uint64_t kBatchSlotCap = 1ull << 30;
 constexpr uint64_t kTailSlots = 1ull << 28;
 // Batched rounds of fewer paths than this run the 3-waves form by default.
 constexpr double kSmallRound = 16.0 * 1024 * 1024;
@@ -131,7 +139,7 @@ struct oxpt_ctx {
     oxpt_ctx *base = nullptr;
     std::shared_ptr<Domain> dom;
     uint32_t n_cu = 0;
-    oxpt_tuning tuning{256, 32, 1u << 30, 0, 0, 0, 0, 0, 0, 0};
+    oxpt_tuning tuning{256, 32, kDefaultMaxPaths, 0, 0, 0, 0, 0, 0, 0};
     uint32_t *d_work = nullptr;
     oxd::Job *d_jobs = nullptr;      // job table; the count sits in d_counters[2 * job_cap]
     uint32_t *d_counters = nullptr;  // two counters per job (pixels, tail paths) + 1 count word
@@ -1097,29 +1105,36 @@ static int ensure_queue(oxpt_ctx *c, uint64_t lanes, uint32_t n_punct) {
 // Work split of one batched round whose jobs are sorted largest first:
 // each job's first tiles are traced as whole-pixel runs and its last
 // tail16/16 of tiles (rounded up) one path per claim, as the persistent
-// megakernel traces a frame. A job keeps pixel runs only when its tail has
-// at least one item per two resident lanes: runs end at times spread over
-// one run's length after the last pixel is claimed, so the lanes done first
-// need about half a run's worth of tail paths each. Smaller jobs are traced
-// path by path throughout (split = 0), jobs of one sample as one-sample
-// runs. Places each job's tail samples in the sample buffer and returns the
-// slots the round needs.
+// megakernel traces a frame. A job keeps pixel runs only when
+//  - its tail has at least one item per two resident lanes: runs end at
+//    times spread over one run's length after the last pixel is claimed,
+//    and a wave stays on the job until its last run ends, so its lanes done
+//    first need about half a run's worth of tail paths each; and
+//  - its tail and the jobs after it hold at least one run's worth of paths
+//    per resident lane: otherwise the launch would end waiting on the runs.
+// Other jobs are traced path by path throughout (split = 0), jobs of one
+// sample as one-sample runs. Places each job's tail samples in the sample
+// buffer and returns the slots the round needs.
 static uint64_t split_round(oxd::Job *jobs, size_t nj, uint32_t tail16, uint64_t lanes) {
-    uint64_t slots = 0;
-    for (size_t k = 0; k < nj; k++) {
+    uint64_t after = 0;  // paths of the jobs after job k
+    for (size_t k = nj; k-- > 0;) {
         oxd::Job &J = jobs[k];
         const uint32_t tiles = J.n_items / 64u;
         uint32_t tail_tiles = tiles;
         if (J.count == 1 || tail16 == 0) {
             tail_tiles = 0;  // one-sample runs, or every pixel a run (OXPT_TAIL_NONE)
         } else if (tail16 < 16) {
-            const uint32_t t = (uint32_t)(((uint64_t)tiles * tail16 + 15u) / 16u);
-            if ((uint64_t)t * 128u >= lanes) tail_tiles = t;
+            const uint64_t t = ((uint64_t)tiles * tail16 + 15u) / 16u;
+            if (t * 128u >= lanes && t * 64u * J.count + after >= (uint64_t)J.count * lanes) tail_tiles = (uint32_t)t;
         }
         J.split = (tiles - tail_tiles) * 64u;
         J.tail_work = tail_tiles * 64u * J.count;
-        J.sample_base = (uint32_t)slots;
-        slots += J.tail_work;
+        after += (uint64_t)J.n_items * J.count;
+    }
+    uint64_t slots = 0;
+    for (size_t k = 0; k < nj; k++) {
+        jobs[k].sample_base = (uint32_t)slots;
+        slots += jobs[k].tail_work;
     }
     return slots;
 }
@@ -1388,9 +1403,15 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         for (size_t k = 0; k < order.size(); k++) sorted[k] = jobs[order[k]];
         return split_round(sorted.data(), sorted.size(), tail16, resident_lanes(c));
     };
-    // One sample slot per tail path, sized for the largest round.
+    // One sample slot per tail path, sized for the largest round; fewer
+    // samples per round while the slots would pass the cap.
     uint64_t slots = 0;
-    for (uint32_t done = 0; done < max_count; done += spd) slots = std::max(slots, build_round(done));
+    for (;;) {
+        slots = 0;
+        for (uint32_t done = 0; done < max_count; done += spd) slots = std::max(slots, build_round(done));
+        if (slots <= kBatchSlotCap || spd == 1) break;
+        spd = (spd + 1) / 2;
+    }
     if (slots >= (1ull << 32)) return fail(c, OXPT_E_ARG, "batch: round too large (lower max_paths)");
     if (int rc = ensure_samples(c, std::max<uint64_t>(slots, 1))) return rc;
     for (uint32_t done = 0; done < max_count; done += spd) {
@@ -1675,9 +1696,9 @@ int oxpt_autotune_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n, uin
     };
     std::vector<Cand> cands;
     for (uint32_t k : {0u, 2u, 3u, 4u, 5u})
-        for (uint32_t mp : {1u << 30, 1u << 28, 1u << 26}) cands.push_back({k, mp, 0u});
-    cands.push_back({0u, 1u << 30, 8u});
-    cands.push_back({0u, 1u << 30, OXPT_STACK_PRIVATE});
+        for (uint32_t mp : {kDefaultMaxPaths, 1u << 28, 1u << 26}) cands.push_back({k, mp, 0u});
+    cands.push_back({0u, kDefaultMaxPaths, 8u});
+    cands.push_back({0u, kDefaultMaxPaths, OXPT_STACK_PRIVATE});
     const oxpt_tuning saved = c->tuning;
     oxpt_tuning best = saved;
     double best_t = 1e300;

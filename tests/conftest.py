@@ -31,6 +31,6 @@ def _restore_gpu_state(request):
     if "gpu" not in request.fixturenames:
         return
     g = request.getfixturevalue("gpu")
-    g.set_tuning(block=256, waves_per_cu=32, max_paths=1 << 30)
+    g.set_tuning(block=256, waves_per_cu=32, max_paths=0xFFFFFFFF)
     g.set_environment(None)
     g.invalidate_scene()
