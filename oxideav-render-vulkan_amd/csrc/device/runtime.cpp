@@ -1372,13 +1372,26 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
     }
     // Rounds: each round adds at most `spd` samples to every item that still
     // has samples left, so one launch stays under max_paths paths.
-    uint64_t total = 0;
     uint32_t max_count = 0;
-    for (size_t i = 0; i < n; i++) {
-        total += (uint64_t)items[i].accum->w * items[i].accum->h;
-        max_count = std::max(max_count, items[i].count);
+    for (size_t i = 0; i < n; i++) max_count = std::max(max_count, items[i].count);
+    // Samples per round: the most that keep the first (largest) round within
+    // max_paths, counting each item's own samples (an item of fewer samples
+    // than the round takes adds only its own).
+    auto round_paths = [&](uint32_t spr) {
+        uint64_t t = 0;
+        for (size_t i = 0; i < n; i++) t += (uint64_t)items[i].accum->w * items[i].accum->h * std::min(spr, items[i].count);
+        return t;
+    };
+    uint32_t spd = 1;
+    for (uint32_t lo = 1, hi = max_count; lo <= hi;) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (round_paths(mid) <= c->tuning.max_paths) {
+            spd = mid;
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
     }
-    uint32_t spd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_count, c->tuning.max_paths / std::max<uint64_t>(total, 1)));
     uint32_t block = c->tuning.block;
     uint32_t grid = std::max(1u, c->n_cu * c->tuning.waves_per_cu / (block / 64u));
     const uint32_t tail16 = c->tuning.tail == 0 ? kDefaultTail : (c->tuning.tail == OXPT_TAIL_NONE ? 0u : c->tuning.tail);
