@@ -337,6 +337,7 @@ def bench_serving(args):
             log(f"{i + 1}/{len(handles)} requests done")
     t1 = time.perf_counter()
     st = srv.stats()
+    _, kern_ms = srv.kernel_time()
     lat_ms = sorted(x * 1e3 for x in lat)
 
     def pct(q, xs=None):
@@ -362,6 +363,7 @@ def bench_serving(args):
                        "refine": {"p50": pct(50, lat_refine), "p99": pct(99, lat_refine)}},
         "busy_mpaths_s": round(st.paths / busy / 1e6, 3) if busy > 0 else None,
         "server": {"ticks": st.ticks, "uploads": st.uploads, "evictions": st.evictions, "busy_s": round(busy, 3),
+                   "kernel_s": round(kern_ms / 1e3, 3),
                    "io_s": round(st.io_s, 3), "failed_ticks": st.failed_ticks, "wall_s": round(t1 - t0, 3),
                    "arrivals_s": round(args.duration, 3), "tick_ms": args.tick_ms,
                    "spp_per_tick": args.spp_tick},

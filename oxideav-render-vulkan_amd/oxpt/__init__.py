@@ -1069,6 +1069,13 @@ class Server:
     def graph_launches(self) -> int:
         return lib.oxpt_ctx_graph_launches(lib.oxpt_server_ctx(self.h))
 
+    def kernel_time(self, reset: bool = False):
+        """(launches, total ms) of the batched steps the server launched, from
+        the HIP events around each launch on its context's stream."""
+        n, ms = u64(), C.c_double()
+        lib.oxpt_ctx_timing(lib.oxpt_server_ctx(self.h), 1 if reset else 0, C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
 
 class RenderRegistry:
     """Name -> factory registry (oxideav_render::RenderRegistry)."""
