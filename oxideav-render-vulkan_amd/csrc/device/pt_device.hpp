@@ -349,6 +349,10 @@ struct Ray {
     uint32_t kx, ky, kz;
     F3 shear;
     bool valid;
+    // 4-wide node rows (float4 index in the node) holding each axis's near
+    // planes: lo (rows 0, 1, 2) where the direction is positive, else hi
+    // (rows 3, 4, 5); the far planes are the other row of the pair.
+    uint32_t nx, ny, nz;
 };
 
 __device__ __forceinline__ Ray make_ray(F3 o, F3 d) {
@@ -376,6 +380,9 @@ __device__ __forceinline__ Ray make_ray(F3 o, F3 d) {
     r.shear = f3(0.0f);
     if (dz != 0.0f) r.shear = {comp(d, kx) / dz, comp(d, ky) / dz, 1.0f / dz};
     r.valid = finite3(o) && finite3(d) && (d.x != 0.0f || d.y != 0.0f || d.z != 0.0f);
+    r.nx = r.inv.x >= 0.0f ? 0u : 3u;
+    r.ny = r.inv.y >= 0.0f ? 1u : 4u;
+    r.nz = r.inv.z >= 0.0f ? 2u : 5u;
     return r;
 }
 
@@ -385,6 +392,21 @@ __device__ __forceinline__ float slab(const Ray &r, float4 lo, float4 hi, float 
     float t0z = (lo.z - r.o.z) * r.inv.z, t1z = (hi.z - r.o.z) * r.inv.z;
     float tnx = fminf(t0x, t1x), tny = fminf(t0y, t1y), tnz = fminf(t0z, t1z);
     float tfx = fmaxf(t0x, t1x), tfy = fmaxf(t0y, t1y), tfz = fmaxf(t0z, t1z);
+    float t0 = fmaxf(fmaxf(0.0f, tnx), fmaxf(tny, tnz));
+    float t1 = fminf(t_max, fminf(tfx, fminf(tfy, tfz)) * 1.0000004f);
+    return t0 <= t1 ? t0 : -1.0f;
+}
+
+// slab() on a box given as its near and far planes along the ray (near =
+// lo where the direction component is positive, else hi): for lo <= hi the
+// entry and exit distances of each axis are already the min and max slab()
+// takes (rounding is monotone and the sign of inv orders them), so the
+// result is slab()'s to the bit with six fewer operations.
+__device__ __forceinline__ float slab_nf(const Ray &r, float nx, float ny, float nz, float fx, float fy, float fz,
+                                         float t_max) {
+    float tnx = (nx - r.o.x) * r.inv.x, tfx = (fx - r.o.x) * r.inv.x;
+    float tny = (ny - r.o.y) * r.inv.y, tfy = (fy - r.o.y) * r.inv.y;
+    float tnz = (nz - r.o.z) * r.inv.z, tfz = (fz - r.o.z) * r.inv.z;
     float t0 = fmaxf(fmaxf(0.0f, tnx), fmaxf(tny, tnz));
     float t1 = fminf(t_max, fminf(tfx, fminf(tfy, tfz)) * 1.0000004f);
     return t0 <= t1 ? t0 : -1.0f;
@@ -653,13 +675,15 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
         count = 0;
     } else if (first != WIDE_EMPTY) {
         const gf4_ptr nd = S.geom + S.wide_base + 8u * first;
-        const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
-        const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
+        // Near and far planes of the four children per axis (the node's lo
+        // and hi rows, ordered by the ray's direction signs).
+        const float4 nx = ldg4(nd, r.nx), ny = ldg4(nd, r.ny), nz = ldg4(nd, r.nz);
+        const float4 fx = ldg4(nd, 3u - r.nx), fy = ldg4(nd, 5u - r.ny), fz = ldg4(nd, 7u - r.nz);
         const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
-        const float s0 = slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f), make_float4(hx.x, hy.x, hz.x, 0.0f), t_max);
-        const float s1 = slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), t_max);
-        const float s2 = slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), t_max);
-        const float s3 = slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), t_max);
+        const float s0 = slab_nf(r, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, t_max);
+        const float s1 = slab_nf(r, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, t_max);
+        const float s2 = slab_nf(r, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, t_max);
+        const float s3 = slab_nf(r, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, t_max);
         // Every child box is tested and empty slots are selected away
         // afterwards (a select is cheaper than a branch per child).
         const float t0 = cn.x != WIDE_EMPTY ? s0 : -1.0f, t1 = cn.y != WIDE_EMPTY ? s1 : -1.0f;
