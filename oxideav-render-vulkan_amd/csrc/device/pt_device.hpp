@@ -675,15 +675,28 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
         count = 0;
     } else if (first != WIDE_EMPTY) {
         const gf4_ptr nd = S.geom + S.wide_base + 8u * first;
-        // Near and far planes of the four children per axis (the node's lo
-        // and hi rows, ordered by the ray's direction signs).
-        const float4 nx = ldg4(nd, r.nx), ny = ldg4(nd, r.ny), nz = ldg4(nd, r.nz);
-        const float4 fx = ldg4(nd, 3u - r.nx), fy = ldg4(nd, 5u - r.ny), fz = ldg4(nd, 7u - r.nz);
         const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
-        const float s0 = slab_nf(r, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, t_max);
-        const float s1 = slab_nf(r, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, t_max);
-        const float s2 = slab_nf(r, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, t_max);
-        const float s3 = slab_nf(r, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, t_max);
+        float s0, s1, s2, s3;
+        if constexpr (!AnyUnordered) {
+            // Near and far planes of the four children per axis (the node's
+            // lo and hi rows, ordered by the ray's direction signs).
+            const float4 nx = ldg4(nd, r.nx), ny = ldg4(nd, r.ny), nz = ldg4(nd, r.nz);
+            const float4 fx = ldg4(nd, 3u - r.nx), fy = ldg4(nd, 5u - r.ny), fz = ldg4(nd, 7u - r.nz);
+            s0 = slab_nf(r, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, t_max);
+            s1 = slab_nf(r, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, t_max);
+            s2 = slab_nf(r, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, t_max);
+            s3 = slab_nf(r, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, t_max);
+        } else {
+            // The NEE-queue form keeps the rows at fixed offsets and slab()'s
+            // min / max: there six per-lane row addresses cost more than the
+            // comparisons they save.
+            const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
+            const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
+            s0 = slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f), make_float4(hx.x, hy.x, hz.x, 0.0f), t_max);
+            s1 = slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), t_max);
+            s2 = slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), t_max);
+            s3 = slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), t_max);
+        }
         // Every child box is tested and empty slots are selected away
         // afterwards (a select is cheaper than a branch per child).
         const float t0 = cn.x != WIDE_EMPTY ? s0 : -1.0f, t1 = cn.y != WIDE_EMPTY ? s1 : -1.0f;
