@@ -580,7 +580,9 @@ int oxpt_packed_info(const oxpt_packed *p, uint32_t *atlas_side, uint32_t *atlas
  * node 0 the root, children the pair (first, first + 1) — the words of
  * src/pathtrace.rs:972-985): 32 words (8 float4) per wide node, to be
  * appended to geom at a float4 index that is a multiple of 8, with
- * Params.offs2[3] set to that index (0 keeps the binary walk). No
+ * Params.offs2[3] set to that index (0 keeps the binary walk). A node's
+ * empty child slots hold count 0xFFFFFFFF and the box lo = +inf,
+ * hi = -inf, which the walk's box test misses by itself. No
  * counterpart in the reference; for `-sys` callers that build their own
  * upload. Two-call: out = NULL returns the node count in *out_vec4.
  * Returns OXPT_E_INVALID when the tree exceeds the wide walk's limits

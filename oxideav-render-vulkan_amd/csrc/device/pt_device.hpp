@@ -686,6 +686,9 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
             s1 = slab_nf(r, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, t_max);
             s2 = slab_nf(r, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, t_max);
             s3 = slab_nf(r, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, t_max);
+            // An empty slot's box is lo = +inf, hi = -inf (bvh.cpp): its near
+            // plane is +inf and its far plane -inf whatever the direction, so
+            // slab_nf misses it by itself.
         } else {
             // The NEE-queue form keeps the rows at fixed offsets and slab()'s
             // min / max: there six per-lane row addresses cost more than the
@@ -696,11 +699,15 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
             s1 = slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), t_max);
             s2 = slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), t_max);
             s3 = slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), t_max);
+            // slab() takes min / max per axis, so an empty slot's inverted box
+            // would pass: empty slots are selected away (a select is cheaper
+            // than a branch per child).
+            s0 = cn.x != WIDE_EMPTY ? s0 : -1.0f;
+            s1 = cn.y != WIDE_EMPTY ? s1 : -1.0f;
+            s2 = cn.z != WIDE_EMPTY ? s2 : -1.0f;
+            s3 = cn.w != WIDE_EMPTY ? s3 : -1.0f;
         }
-        // Every child box is tested and empty slots are selected away
-        // afterwards (a select is cheaper than a branch per child).
-        const float t0 = cn.x != WIDE_EMPTY ? s0 : -1.0f, t1 = cn.y != WIDE_EMPTY ? s1 : -1.0f;
-        const float t2 = cn.z != WIDE_EMPTY ? s2 : -1.0f, t3 = cn.w != WIDE_EMPTY ? s3 : -1.0f;
+        const float t0 = s0, t1 = s1, t2 = s2, t3 = s3;
         if (AnyUnordered && any) {
             // Any-hit query: t_max stays put until the walk ends at the
             // first accepted hit, so the order children are visited in
