@@ -67,7 +67,7 @@ def workload(name):
 
 # Committed PMC readings per workload: file and the default form they were taken on.
 PMC_FILES = {
-    "c1": ("trace_persistent_c1_pmc.json", "trace_persistent<5,early,opaque>"),  # no punctual lights
+    "c1": ("trace_persistent_c1_pmc.json", "trace_persistent<6,early,opaque>"),  # no punctual lights
     "c2": ("trace_persistent_c2_pmc.json", "trace_persistent<4,queue>"),  # 16 punctual lights, alpha blending
     "c0": ("trace_batch_c0_pmc.json", "trace_batch<3,early,opaque>"),  # the one-job batch of a small frame
 }

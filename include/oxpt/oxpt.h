@@ -291,7 +291,7 @@ typedef struct oxpt_tuning {
     uint32_t waves_per_cu;   /* persistent residency: workgroups = CUs * this / waves per WG */
     uint32_t max_paths;      /* paths per launch cap (samples are split, order kept); default 2^32 - 1 */
     uint32_t kernel;         /* 0 = default: oxpt_refine runs the persistent megakernel at 4 waves per SIMD,
-                                5 for scenes without punctual lights (the early-continuation build; frames
+                                6 for scenes without punctual lights (the early-continuation build; frames
                                 with fewer pixels than resident lanes: the one-job batched kernel),
                                 oxpt_refine_batch the batched (one path per work item) megakernel at 4,
                                 5 when no job has punctual lights (3 for rounds of fewer than 2^24 paths);

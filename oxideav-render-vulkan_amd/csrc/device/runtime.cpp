@@ -38,8 +38,11 @@ constexpr uint64_t kBatchSlotCap = 1ull << 30;
 constexpr uint64_t kTailSlots = 1ull << 28;
 // Batched rounds of fewer paths than this run the 3-waves form by default.
 constexpr double kSmallRound = 16.0 * 1024 * 1024;
-// Default form of a refine without punctual lights (early continuation):
-// the persistent megakernel built for 5 waves per SIMD.
+// Default forms without punctual lights (early continuation): the
+// persistent megakernel of a refine built for 6 waves per SIMD (the 1080p
+// frame: 5 waves 1021, 6 waves 1032 Mpaths/s), the batched one for 5 (the
+// uneven batch: 5 waves 1041, 6 waves 860).
+constexpr uint32_t kEarlyRefineForm = 5;
 constexpr uint32_t kEarlyDefaultForm = 4;
 // Default form of a refine with 1..31 punctual lights (early continuation
 // with a per-lane NEE queue in device memory).
@@ -1244,7 +1247,7 @@ int oxpt_refine(oxpt_ctx *c, const oxpt_scene *s, oxpt_accum *a, const oxpt_para
     const uint32_t mode = trace_mode(c->tuning, P->cfg[2]);
     const bool opaque = mode != 0 && opaque_walk(c->tuning, s->host_tables, *P);
     oxpt_tuning form = c->tuning;
-    if (form.kernel == 0 && mode == 1) form.kernel = kEarlyDefaultForm;
+    if (form.kernel == 0 && mode == 1) form.kernel = kEarlyRefineForm;
     if (form.kernel == 0 && mode == 2) form.kernel = kQueueDefaultForm;
     if (mode == 2)
         if (int rc = ensure_queue(c, (uint64_t)grid * block, P->cfg[2])) return rc;
