@@ -72,14 +72,18 @@ uint32_t form_waves(uint32_t kernel) {
 }
 
 // Traversal-stack entries per lane held in LDS. Default: as many as leave
-// every resident workgroup room on a CU (160 KiB of LDS), at most 16.
+// every resident workgroup room on a CU (160 KiB of LDS), at most 16. A
+// workgroup's LDS is allocated in granules, with its dynamic part aligned
+// after the static one: 1 KiB per workgroup is kept free for that, since a
+// workgroup a few hundred bytes short of its share lowers the waves per
+// SIMD the form was built for.
 uint32_t stack_lds_entries(const oxpt_tuning &t, uint32_t block, bool batched) {
     if (t.stack_lds == OXPT_STACK_PRIVATE) return 0;
     if (t.stack_lds != 0) return t.stack_lds;
     const uint32_t waves_cu = 4u * form_waves(t.kernel);
     const uint32_t wg_cu = std::max(1u, waves_cu / (block / 64u));
     const uint32_t per_wg = 160u * 1024u / wg_cu;
-    const uint32_t fixed = kTableLds + (batched ? 0u : kCdfLds);
+    const uint32_t fixed = kTableLds + (batched ? 0u : kCdfLds) + 1024u;
     if (per_wg <= fixed) return 0;
     return std::min(16u, (per_wg - fixed) / ((block / 64u) * 512u));
 }
