@@ -164,6 +164,10 @@ struct Scene {
     // (one float per emissive triangle).
     const float *lds_marg;
     const float *lds_ecdf;
+    // geom + tri_base and geom + wide_base: the walk's loads index these
+    // directly, so the offsets need not stay live beside the base pointer.
+    gf4_ptr tris;
+    gf4_ptr wide;
 };
 
 __device__ __forceinline__ float tf(const Scene &S, uint32_t i) { return bitsf(S.tables[i]); }
@@ -171,7 +175,7 @@ __device__ __forceinline__ F3 tv3(const Scene &S, uint32_t i) { return {tf(S, i)
 __device__ __forceinline__ float4 tv4(const Scene &S, uint32_t i) {
     return make_float4(tf(S, i), tf(S, i + 1), tf(S, i + 2), tf(S, i + 3));
 }
-__device__ __forceinline__ float4 tri_word(const Scene &S, uint32_t slot, uint32_t v) { return ldg4(S.geom, S.tri_base + 3u * slot + v); }
+__device__ __forceinline__ float4 tri_word(const Scene &S, uint32_t slot, uint32_t v) { return ldg4(S.tris, 3u * slot + v); }
 __device__ __forceinline__ F3 tri_pos(const Scene &S, uint32_t slot, uint32_t v) { return xyz(tri_word(S, slot, v)); }
 __device__ __forceinline__ uint32_t tri_global(const Scene &S, uint32_t slot) { return fbits(tri_word(S, slot, 0).w); }
 __device__ __forceinline__ uint32_t tri_light(const Scene &S, uint32_t slot) { return fbits(tri_word(S, slot, 1).w); }
@@ -674,7 +678,7 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
         first = WIDE_EMPTY;
         count = 0;
     } else if (first != WIDE_EMPTY) {
-        const gf4_ptr nd = S.geom + S.wide_base + 8u * first;
+        const gf4_ptr nd = S.wide + 8u * first;
         const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
         float s0, s1, s2, s3;
         if constexpr (!AnyUnordered) {

@@ -23,6 +23,8 @@ __device__ __forceinline__ Scene make_scene(const oxpt_params &P, const float4 *
     S.atlas_side = atlas_side;
     S.tri_base = P.offs2[2];
     S.wide_base = P.offs2[3];
+    S.tris = S.geom + S.tri_base;
+    S.wide = S.geom + S.wide_base;
     S.node_count = P.env[3];
     S.off_mat = P.offs0[2];
     S.off_plights = P.offs0[3];

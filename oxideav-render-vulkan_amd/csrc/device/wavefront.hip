@@ -45,6 +45,8 @@ __device__ __forceinline__ Scene job_scene(const Job &J, const float *lds_sheen)
     S.atlas_side = J.atlas_side;
     S.tri_base = J.P.offs2[2];
     S.wide_base = J.P.offs2[3];
+    S.tris = S.geom + S.tri_base;
+    S.wide = S.geom + S.wide_base;
     S.node_count = J.P.env[3];
     S.off_mat = J.P.offs0[2];
     S.off_plights = J.P.offs0[3];
