@@ -424,8 +424,8 @@ struct Hit {
 };
 
 // Woop-Benthin-Wald watertight test on a triangle already in registers.
-__device__ __forceinline__ bool intersect_tri(const Ray &r, F3 p0, F3 p1, F3 p2, float t_max, float &t_out, F3 &b_out,
-                                              bool &front_out) {
+__device__ __forceinline__ bool intersect_tri(const Ray &r, F3 p0, F3 p1, F3 p2, float t_max, float &t_out,
+                                              F3 &b_out) {
     F3 a = p0 - r.o, b = p1 - r.o, c = p2 - r.o;
     float akz = comp(a, r.kz), bkz = comp(b, r.kz), ckz = comp(c, r.kz);
     float ax = comp(a, r.kx) - r.shear.x * akz;
@@ -447,8 +447,24 @@ __device__ __forceinline__ bool intersect_tri(const Ray &r, F3 p0, F3 p1, F3 p2,
     float u = vv * rcp, v = ww * rcp;
     t_out = t;
     b_out = {1.0f - u - v, u, v};
-    front_out = dot(r.d, cross(p1 - p0, p2 - p0)) < 0.0f;
     return true;
+}
+
+// Whether the ray meets the triangle's front face (wgsl:617, with the
+// intersection above). The walks evaluate it per accepted candidate; built
+// with Defer, only for camera rays (whose acceptance culls back faces), and
+// for other closest-hit rays once, for the hit the walk returns (any-hit
+// queries do not use it). Deferring saves the batched form work; the
+// single-frame form at 6 waves per SIMD spills more with it (the 1080p
+// frame 1027 -> 925 Mpaths/s, the uneven batch 1053 -> 1062).
+__device__ __forceinline__ bool tri_front(const Ray &r, F3 p0, F3 p1, F3 p2) {
+    return dot(r.d, cross(p1 - p0, p2 - p0)) < 0.0f;
+}
+// The front flag of the hit a closest-hit walk returns (camera rays have it
+// from their acceptance test already).
+__device__ __forceinline__ void set_front(const Scene &S, const Ray &r, bool primary, bool any, Hit &best) {
+    if (best.slot != NONE && !primary && !any)
+        best.front = tri_front(r, tri_pos(S, best.slot, 0), tri_pos(S, best.slot, 1), tri_pos(S, best.slot, 2));
 }
 
 // Any-hit acceptance (wgsl:622-636) given the material flags word.
@@ -505,7 +521,7 @@ __device__ __forceinline__ uint64_t stack_entry(float t_far, uint32_t first, uin
 // count) travels with it — read from the parent's child pair on descent,
 // from the stack entry on a pop — so each inner-node step issues only the
 // one 64-byte child-pair load.
-template <int LdsDepth, bool Opaque = false>
+template <int LdsDepth, bool Opaque = false, bool Defer = false>
 __device__ __forceinline__ Hit traverse2(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                          uint32_t ray_id, bool primary, bool any, uint32_t skip, lds_u64 *lds_stack,
                                          uint32_t lds_depth) {
@@ -536,10 +552,10 @@ __device__ __forceinline__ Hit traverse2(const Scene &S, const Ray &r, float t_m
                 float4 w0 = tri_word(S, slot, 0), w1 = tri_word(S, slot, 1), w2 = tri_word(S, slot, 2);
                 float t;
                 F3 b;
-                bool front;
-                if (!intersect_tri(r, xyz(w0), xyz(w1), xyz(w2), t_max, t, b, front)) continue;
+                if (!intersect_tri(r, xyz(w0), xyz(w1), xyz(w2), t_max, t, b)) continue;
                 uint32_t gid = fbits(w0.w);
                 if (gid == skip) continue;
+                const bool front = (!Defer || primary) && tri_front(r, xyz(w0), xyz(w1), xyz(w2));
                 if (!accept_hit<Opaque>(S, slot, b, front, fbits(w2.w), path_seed, ray_id, primary, gid)) continue;
                 best.slot = slot;
                 best.t = t;
@@ -602,12 +618,13 @@ __device__ __forceinline__ Hit traverse2(const Scene &S, const Ray &r, float t_m
         }
         if (!found) break;
     }
+    if constexpr (Defer) set_front(S, r, primary, any, best);
     return best;
 }
 
 // Leaf: test slots [first, first + count) in order (wgsl:683-699); returns
 // true when an any-hit query is answered.
-template <bool Opaque>
+template <bool Opaque, bool Defer = false>
 __device__ __forceinline__ bool leaf_hits(const Scene &S, const Ray &r, uint32_t first, uint32_t count, float &t_max,
                                           Hit &best, uint32_t path_seed, uint32_t ray_id, bool primary, bool any,
                                           uint32_t skip) {
@@ -615,10 +632,10 @@ __device__ __forceinline__ bool leaf_hits(const Scene &S, const Ray &r, uint32_t
         float4 w0 = tri_word(S, slot, 0), w1 = tri_word(S, slot, 1), w2 = tri_word(S, slot, 2);
         float t;
         F3 b;
-        bool front;
-        if (!intersect_tri(r, xyz(w0), xyz(w1), xyz(w2), t_max, t, b, front)) continue;
+        if (!intersect_tri(r, xyz(w0), xyz(w1), xyz(w2), t_max, t, b)) continue;
         uint32_t gid = fbits(w0.w);
         if (gid == skip) continue;
+        const bool front = (!Defer || primary) && tri_front(r, xyz(w0), xyz(w1), xyz(w2));
         if (!accept_hit<Opaque>(S, slot, b, front, fbits(w2.w), path_seed, ray_id, primary, gid)) continue;
         best.slot = slot;
         best.t = t;
@@ -657,7 +674,7 @@ constexpr uint32_t WIDE_EMPTY = 0xffffffffu;
 // AnyUnordered: any-hit queries skip the ordering (the answer is the same;
 // faster where shadow rays to punctual lights dominate, slower on open
 // scenes whose shadow rays find near occluders sooner in distance order).
-template <bool AnyUnordered = false, bool Opaque = false>
+template <bool AnyUnordered = false, bool Opaque = false, bool Defer = false>
 __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t_max, Hit &best, uint32_t &first,
                                           uint32_t &count, uint32_t &lf, uint32_t &lc, uint32_t &sp,
                                           uint32_t path_seed, uint32_t ray_id, bool primary, bool any, uint32_t skip,
@@ -665,7 +682,7 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
     // The pending leaf's next triangle (its triangles in slot order, as
     // leaf_hits takes them, one per step).
     if (lc > 0) {
-        if (leaf_hits<Opaque>(S, r, lf, 1, t_max, best, path_seed, ray_id, primary, any, skip)) return false;
+        if (leaf_hits<Opaque, Defer>(S, r, lf, 1, t_max, best, path_seed, ray_id, primary, any, skip)) return false;
         lf++;
         lc--;
     }
@@ -806,7 +823,7 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
     return true;
 }
 
-template <int LdsDepth, bool AnyUnordered = false, bool Opaque = false>
+template <int LdsDepth, bool AnyUnordered = false, bool Opaque = false, bool Defer = false>
 __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                          uint32_t ray_id, bool primary, bool any, uint32_t skip, lds_u64 *lds_stack,
                                          uint32_t lds_depth) {
@@ -824,22 +841,23 @@ __device__ __forceinline__ Hit traverse4(const Scene &S, const Ray &r, float t_m
     constexpr uint32_t priv_off = LdsDepth > 0 ? (uint32_t)LdsDepth : 0u;
     uint32_t sp = 0, first = 0, count = 0;  // wide node 0, inner
     uint32_t lf = 0, lc = 0;                // no pending leaf
-    while (wide_step<AnyUnordered, Opaque>(S, r, t_max, best, first, count, lf, lc, sp, path_seed, ray_id, primary, any, skip,
+    while (wide_step<AnyUnordered, Opaque, Defer>(S, r, t_max, best, first, count, lf, lc, sp, path_seed, ray_id, primary, any, skip,
                                    lds, priv, in_lds, priv_off)) {
     }
+    if constexpr (Defer) set_front(S, r, primary, any, best);
     return best;
 }
 
 // Closest (or, with `any`, some) accepted hit of the ray up to t_max_in:
 // the 4-wide walk when the scene carries one, else the binary walk.
-template <int LdsDepth = 0, bool AnyUnordered = false, bool Opaque = false>
+template <int LdsDepth = 0, bool AnyUnordered = false, bool Opaque = false, bool Defer = false>
 __device__ __forceinline__ Hit traverse(const Scene &S, const Ray &r, float t_max_in, uint32_t path_seed,
                                         uint32_t ray_id, bool primary, bool any, uint32_t skip,
                                         lds_u64 *lds_stack = nullptr, uint32_t lds_depth = 0) {
     if (S.wide_base != 0)
-        return traverse4<LdsDepth, AnyUnordered, Opaque>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack,
+        return traverse4<LdsDepth, AnyUnordered, Opaque, Defer>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack,
                                                  lds_depth);
-    return traverse2<LdsDepth, Opaque>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack, lds_depth);
+    return traverse2<LdsDepth, Opaque, Defer>(S, r, t_max_in, path_seed, ray_id, primary, any, skip, lds_stack, lds_depth);
 }
 
 __device__ __forceinline__ float offset_one(float p, float n) {

@@ -575,13 +575,13 @@ __device__ bool advance(const oxpt_params &P, const Scene &S, const uint32_t *ld
 // `lds_stack` / `lds_depth`: this wave's LDS share of the traversal stack
 // (depth 0: the whole stack in private memory). Opaque: the walk built for
 // scenes whose materials are all alpha mode OPAQUE (accept_hit).
-template <int Mode = 0, bool Opaque = false>
+template <int Mode = 0, bool Opaque = false, bool Defer = false>
 __device__ __forceinline__ bool step_path(const oxpt_params &P, const Scene &S, const uint32_t *lds_sobol,
                                           PathState &st, bool *covered, lds_u64 *lds_stack = nullptr,
                                           uint32_t lds_depth = 0, const NeeQueue &Q = NeeQueue{nullptr, 0, 0}) {
     // The NEE-queue form (punctual lights: most rays are shadow rays) walks
     // any-hit queries without ordering the children.
-    Hit hit = traverse<-1, Mode == 2, Opaque>(S, make_ray(st.r_o, st.r_d), st.r_tmax, st.path_seed, st.r_id,
+    Hit hit = traverse<-1, Mode == 2, Opaque, Defer>(S, make_ray(st.r_o, st.r_d), st.r_tmax, st.path_seed, st.r_id,
                                       !st.shadow && st.k == 0, st.shadow, st.r_skip, lds_stack, lds_depth);
     return advance<Mode>(P, S, lds_sobol, st, hit, covered, Q);
 }
@@ -842,7 +842,7 @@ trace_batch(BatchArgs B, const Job *__restrict__ jobs, uint32_t lds_depth) {
             // scenes 874 -> 952 Mpaths/s, on a 256x256 room at 16 spp 688 ->
             // 905).
             bool covered, ended;
-            while (!(ended = step_path<Mode, Opaque>(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth, Q)) &&
+            while (!(ended = step_path<Mode, Opaque, true>(J.P, S, lds_sobol, st, &covered, lds_stack, lds_depth, Q)) &&
                    st.shadow && (Mode == 1 || J.P.cfg[2] != 0)) {
             }
             if (ended) {
