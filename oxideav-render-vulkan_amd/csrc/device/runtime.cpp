@@ -39,11 +39,11 @@ constexpr uint64_t kTailSlots = 1ull << 28;
 // Batched rounds of fewer paths than this run the 3-waves form by default.
 constexpr double kSmallRound = 16.0 * 1024 * 1024;
 // Default forms without punctual lights (early continuation): the
-// persistent megakernel of a refine built for 6 waves per SIMD (the 1080p
-// frame: 5 waves 1021, 6 waves 1032 Mpaths/s), the batched one for 5 (the
-// uneven batch: 5 waves 1041, 6 waves 860).
+// persistent megakernel of a refine and the batched megakernel, both built
+// for 6 waves per SIMD (the 1080p frame: 5 waves 1021, 6 waves 1032
+// Mpaths/s; the uneven batch of 64 scenes: 5 waves 1063, 6 waves 1073).
 constexpr uint32_t kEarlyRefineForm = 5;
-constexpr uint32_t kEarlyDefaultForm = 4;
+constexpr uint32_t kEarlyDefaultForm = 5;
 // Default form of a refine with 1..31 punctual lights (early continuation
 // with a per-lane NEE queue in device memory).
 constexpr uint32_t kQueueDefaultForm = 0;
