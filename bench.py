@@ -180,9 +180,12 @@ def bench_uneven_batch(args):
     tuned = None
     if args.kernel == 0:
         dev.set_tuning(kernel=0, max_paths=0xFFFFFFFF)
-        tuned, tune_ms = dev.autotune_batch(items, spp_per_trial=16)
+        # 64-spp trials: at 16 the jobs' work mix (most paths in the 256-spp
+        # jobs) is not the full run's, and the trials picked the 5-wave form
+        # where the full run is faster at 6.
+        tuned, tune_ms = dev.autotune_batch(items, spp_per_trial=64)
         log(f"autotune_batch: kernel {tuned.kernel}, max_paths {tuned.max_paths}, stack_lds {tuned.stack_lds} "
-            f"({tune_ms:.1f} ms per 16-spp trial)")
+            f"({tune_ms:.1f} ms per 64-spp trial)")
 
     def batched(graph):
         def run():
