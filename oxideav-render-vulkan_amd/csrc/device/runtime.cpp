@@ -36,8 +36,14 @@ constexpr uint32_t kDefaultMaxPaths = 0xFFFFFFFFu;
 // samples per item are halved until its slots fit this cap (16 GiB).
 constexpr uint64_t kBatchSlotCap = 1ull << 30;
 constexpr uint64_t kTailSlots = 1ull << 28;
-// Batched rounds of fewer paths than this run the 3-waves form by default.
+// Batched rounds of fewer paths than kSmallRound run the 3-waves form by
+// default, where the end of the round (one path's latency) weighs more than
+// throughput; without punctual lights, rounds from kTinyRound up run the
+// 5-waves form (a 256x256 frame at 16 spp, 1 M paths: 3 waves 923, 5 waves
+// 848 Mpaths/s; serving ticks of about 8 M paths: 5 waves 794, 3 waves 779,
+// 6 waves 756 Mpaths/s of busy time).
 constexpr double kSmallRound = 16.0 * 1024 * 1024;
+constexpr double kTinyRound = 4.0 * 1024 * 1024;
 // Default forms without punctual lights (early continuation): the
 // persistent megakernel of a refine and the batched megakernel, both built
 // for 6 waves per SIMD (the 1080p frame: 5 waves 1021, 6 waves 1032
@@ -1449,7 +1455,7 @@ int oxpt_refine_batch(oxpt_ctx *c, const oxpt_batch_item *items, size_t n) {
         for (size_t i = 0; i < n && opaque; i++)
             if (items[i].count > done) opaque = opaque_walk(c->tuning, items[i].scene->host_tables, items[i].params);
         oxpt_tuning form = c->tuning;
-        if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = 2;
+        if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = mode == 1 && round_paths >= kTinyRound ? 4 : 2;
         else if (form.kernel == 0 && mode == 1) form.kernel = kEarlyDefaultForm;
         else if (form.kernel == 0 && mode == 2) form.kernel = kQueueDefaultForm;
         if (mode == 2)
@@ -1598,7 +1604,7 @@ int oxpt_refine_batch_noalloc(oxpt_ctx *c, const oxpt_batch_item *items, size_t 
         for (size_t i = 0; i < n && opaque; i++)
             if (items[i].count > done) opaque = opaque_walk(c->tuning, items[i].scene->host_tables, items[i].params);
         oxpt_tuning form = c->tuning;
-        if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = 2;
+        if (form.kernel == 0 && round_paths < kSmallRound) form.kernel = mode == 1 && round_paths >= kTinyRound ? 4 : 2;
         else if (form.kernel == 0 && mode == 1) form.kernel = kEarlyDefaultForm;
         const uint32_t lds = stack_lds_entries(form, block, true);
         hipError_t e = hipMemcpyAsync(djobs, jobs, nj * sizeof(oxd::Job), hipMemcpyHostToDevice, c->stream);
