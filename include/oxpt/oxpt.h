@@ -294,7 +294,8 @@ typedef struct oxpt_tuning {
                                 6 for scenes without punctual lights (the early-continuation build; frames
                                 with fewer pixels than resident lanes: the one-job batched kernel),
                                 oxpt_refine_batch the batched (one path per work item) megakernel at 4,
-                                6 when no job has punctual lights (3 for rounds of fewer than 2^24 paths);
+                                6 when no job has punctual lights (rounds of fewer than 2^24 paths: 3, or
+                                5 from 2^22 paths up when no job has punctual lights);
                                 1 = plain one-pixel-per-thread, 2..6 = persistent built for 3, 4, 5, 6, 8
                                 waves per SIMD, 7 = wavefront (separate shade / extend / occlude kernels),
                                 8 = persistent built for 2 waves per SIMD */
