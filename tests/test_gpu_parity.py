@@ -455,3 +455,51 @@ def test_opaque_walk_bitwise(gpu):
     dev.set_tuning()
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def _back_face_scene():
+    """A lit back wall facing the camera and, in front of it, a single-sided
+    red quad turned away from the camera: camera rays pass through the quad
+    (acceptance culls back faces of camera rays on single-sided materials,
+    wgsl:625), while bounce and shadow rays hit it (its shadow and its red
+    light on the wall)."""
+    s = oxpt.Scene()
+    wall = s.add_material(ptscenes.lambert((0.8, 0.8, 0.8)))
+    pos, uv = ptscenes.quad([-3, -2, -2], [3, -2, -2], [3, 2, -2], [-3, 2, -2])  # normal +z
+    s.add_triangles(pos, wall, uv0=uv)
+    red = s.add_material(ptscenes.lambert((0.9, 0.1, 0.1), double_sided=0))
+    pos, uv = ptscenes.quad([-1, -1, 0], [-1, 1, 0], [1, 1, 0], [1, -1, 0])  # normal -z
+    s.add_triangles(pos, red, uv0=uv)
+    light = s.add_material(ptscenes.lambert((0, 0, 0), emissive=(8.0, 8.0, 8.0)))
+    pos, uv = ptscenes.quad([-0.5, 2.5, -0.5], [0.5, 2.5, -0.5], [0.5, 2.5, 0.5], [-0.5, 2.5, 0.5])
+    s.add_triangles(pos, light)  # normal -y: facing down
+    s.set_camera((0.3, 0.2, 3.0), (0.0, 0.0, -2.0), yfov=0.9)
+    return s
+
+
+def test_camera_rays_cull_single_sided_back_faces(gpu):
+    """The batched form tests a candidate's front face only for camera rays
+    and sets it once for the hit a walk returns; the single-frame forms test
+    it per candidate. Both equal the plain kernel bit for bit on a scene
+    where camera rays meet a single-sided back face and bounce rays hit it,
+    and match the oracle."""
+    scene = _back_face_scene()
+    opts = oxpt.RenderOptions(width=96, height=64, samples_per_pixel=8, max_bounces=6)
+    gpu.set_environment(None)
+    gpu.set_tuning(kernel=1)
+    ref = gpu.render_hdr(scene, opts).view(np.uint32).copy()
+    assert gpu.last_kernel() == "trace_tiles", gpu.last_kernel()
+    for kernel, form in ((0, "trace_batch"), (4, "trace_persistent"), (5, "trace_persistent")):
+        gpu.set_tuning(kernel=kernel)
+        got = gpu.render_hdr(scene, opts).view(np.uint32)
+        assert gpu.last_kernel().startswith(form), (kernel, gpu.last_kernel())
+        assert np.array_equal(got, ref), f"kernel {kernel}"
+    gpu.set_tuning(kernel=0)
+    gh = gpu.render_hdr(scene, opts)
+    gi = gpu.image()
+    # The quad is seen through: the centre of the frame is the lit wall, not red.
+    h, w = gh.shape[:2]
+    c = gh[h // 2 - 4:h // 2 + 4, w // 2 - 4:w // 2 + 4, :3].mean(axis=(0, 1))
+    assert c[0] < 3.0 * c[1] + 0.05, c
+    _, ci, ch = oracle_frame(scene, opts)
+    parity("back faces", gh, gi, ch, ci, 60.0)
