@@ -697,38 +697,17 @@ __device__ __forceinline__ bool wide_step(const Scene &S, const Ray &r, float &t
     } else if (first != WIDE_EMPTY) {
         const gf4_ptr nd = S.wide + 8u * first;
         const uint4 rf = ldgu4((gu32_ptr)(nd + 6)), cn = ldgu4((gu32_ptr)(nd + 7));
-        float s0, s1, s2, s3;
-        if constexpr (!AnyUnordered) {
-            // Near and far planes of the four children per axis (the node's
-            // lo and hi rows, ordered by the ray's direction signs).
-            const float4 nx = ldg4(nd, r.nx), ny = ldg4(nd, r.ny), nz = ldg4(nd, r.nz);
-            const float4 fx = ldg4(nd, 3u - r.nx), fy = ldg4(nd, 5u - r.ny), fz = ldg4(nd, 7u - r.nz);
-            s0 = slab_nf(r, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, t_max);
-            s1 = slab_nf(r, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, t_max);
-            s2 = slab_nf(r, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, t_max);
-            s3 = slab_nf(r, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, t_max);
-            // An empty slot's box is lo = +inf, hi = -inf (bvh.cpp): its near
-            // plane is +inf and its far plane -inf whatever the direction, so
-            // slab_nf misses it by itself.
-        } else {
-            // The NEE-queue form keeps the rows at fixed offsets and slab()'s
-            // min / max: there six per-lane row addresses cost more than the
-            // comparisons they save.
-            const float4 lx = ldg4(nd, 0), ly = ldg4(nd, 1), lz = ldg4(nd, 2);
-            const float4 hx = ldg4(nd, 3), hy = ldg4(nd, 4), hz = ldg4(nd, 5);
-            s0 = slab(r, make_float4(lx.x, ly.x, lz.x, 0.0f), make_float4(hx.x, hy.x, hz.x, 0.0f), t_max);
-            s1 = slab(r, make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), t_max);
-            s2 = slab(r, make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), t_max);
-            s3 = slab(r, make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), t_max);
-            // slab() takes min / max per axis, so an empty slot's inverted box
-            // would pass: empty slots are selected away (a select is cheaper
-            // than a branch per child).
-            s0 = cn.x != WIDE_EMPTY ? s0 : -1.0f;
-            s1 = cn.y != WIDE_EMPTY ? s1 : -1.0f;
-            s2 = cn.z != WIDE_EMPTY ? s2 : -1.0f;
-            s3 = cn.w != WIDE_EMPTY ? s3 : -1.0f;
-        }
-        const float t0 = s0, t1 = s1, t2 = s2, t3 = s3;
+        // Near and far planes of the four children per axis (the node's lo
+        // and hi rows, ordered by the ray's direction signs). An empty
+        // slot's box is lo = +inf, hi = -inf (bvh.cpp): its near plane is
+        // +inf and its far plane -inf whatever the direction, so slab_nf
+        // misses it by itself.
+        const float4 nx = ldg4(nd, r.nx), ny = ldg4(nd, r.ny), nz = ldg4(nd, r.nz);
+        const float4 fx = ldg4(nd, 3u - r.nx), fy = ldg4(nd, 5u - r.ny), fz = ldg4(nd, 7u - r.nz);
+        const float t0 = slab_nf(r, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, t_max);
+        const float t1 = slab_nf(r, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, t_max);
+        const float t2 = slab_nf(r, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, t_max);
+        const float t3 = slab_nf(r, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, t_max);
         if (AnyUnordered && any) {
             // Any-hit query: t_max stays put until the walk ends at the
             // first accepted hit, so the order children are visited in
