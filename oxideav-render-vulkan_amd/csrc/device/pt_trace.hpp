@@ -704,12 +704,15 @@ trace_persistent(TraceArgs args, uint32_t lds_depth) {
             init_path(A.P, lds_sobol, px, py, A.first + done, st);
             need_init = false;
         }
-        // With punctual lights a hit has many NEE shadow rays: trace them
-        // before returning to the claim step, so the lanes of a wave stay in
-        // the same phase of their bounces (less divergent shading).
+        // A hit's shadow rays are traced before the wave returns to the claim
+        // step, so the lanes of a wave stay in the same phase of their
+        // bounces (less divergent shading): with punctual lights (many
+        // shadow rays per hit) and in the early form without them (lanes
+        // whose hit made no shadow ray would otherwise shade a step ahead;
+        // the 1080p frame 1042 -> 1055 Mpaths/s).
         bool covered, ended;
         while (!(ended = step_path<Mode, Opaque>(A.P, S, lds_sobol, st, &covered, lds_stack, lds_depth, Q)) && st.shadow &&
-               A.P.cfg[2] != 0) {
+               (Mode == 1 || A.P.cfg[2] != 0)) {
         }
         if (ended) {
             if (to_buffer) {
